@@ -1,0 +1,230 @@
+"""FP64 CPU oracle for meshoid's neighbour-weighted operator path.
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, by ``__graft_entry__.smoke()``
+(as the checker) and by the ``cpu_baseline`` leg of ``bench.py``.  The product
+package ``meshoid_amd`` never imports it.
+
+Each function restates the reference code it cites (paths relative to the
+meshoid source tree):
+
+* :func:`knn`                -- ``cKDTree(pos, boxsize).query(x, k)``, the call at
+  ``src/meshoid/Meshoid.py:145,157`` (exact kNN, self included, ascending).
+* :func:`hsml_iter`          -- ``HsmlIter``, ``src/meshoid/kernel_density.py:14-57``.
+* :func:`kernel`             -- ``Kernel``, ``src/meshoid/kernel_density.py:82-96``.
+* :func:`gradient_weights`   -- ``gradient_weights``, ``src/meshoid/derivatives.py:117-138``.
+* :func:`apply_weights`      -- the einsum of ``Meshoid.D``/``D2``,
+  ``src/meshoid/Meshoid.py:258,268,291,295``.
+* :class:`OracleMeshoid`     -- the ``Meshoid`` state machine for the hot-path
+  methods (``Meshoid.py:15-105,110-170,237-331``).
+
+The numerical core is ``meshoid_oracle.c`` (strict FP64, no FMA contraction),
+built into ``liboracle.so`` next to this file.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile liboracle.so in place (gcc, see Makefile)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_knn.argtypes = [_f64p, ctypes.c_int64, ctypes.c_int, ctypes.c_double, _f64p,
+                              ctypes.c_int64, ctypes.c_int, _i64p, _f64p]
+        L.orc_knn_brute.argtypes = L.orc_knn.argtypes
+        L.orc_hsml_iter.argtypes = [_f64p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_int, _f64p, _i32p]
+        L.orc_kernel.argtypes = [ctypes.c_double]
+        L.orc_kernel.restype = ctypes.c_double
+        L.orc_gradient_weights.argtypes = [_f64p, ctypes.c_int64, ctypes.c_int, _i64p,
+                                           ctypes.c_int64, ctypes.c_int, _f64p, _i64p,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p,
+                                           _i32p]
+        L.orc_apply_weights.argtypes = [_f64p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, _i64p, _i64p, _f64p,
+                                        ctypes.c_int, _f64p]
+        L.orc_num_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+def _pos2d(pos):
+    pos = np.asarray(pos, dtype=np.float64)
+    if pos.ndim == 1:
+        pos = pos[:, None]
+    return np.ascontiguousarray(pos)
+
+
+def knn(pos, k, boxsize=None, queries=None, brute=False):
+    """Exact kNN (f64).  Returns (dist (nq,k) f64, ngb (nq,k) int64)."""
+    pos = _pos2d(pos)
+    n, dim = pos.shape
+    q = pos if queries is None else _pos2d(queries)
+    nq = q.shape[0]
+    ngb = np.empty((nq, k), dtype=np.int64)
+    dist = np.empty((nq, k), dtype=np.float64)
+    fn = lib().orc_knn_brute if brute else lib().orc_knn
+    rc = fn(pos, n, dim, float(boxsize) if boxsize else -1.0, q, nq, int(k), ngb, dist)
+    if rc != 0:
+        raise ValueError("oracle knn failed")
+    return dist, ngb
+
+
+def hsml_iter(neighbor_dists, dim=3, error_norm=1e-6, max_iter=1000, return_iters=False):
+    d = np.ascontiguousarray(neighbor_dists, dtype=np.float64)
+    n, k = d.shape
+    h = np.empty(n, dtype=np.float64)
+    it = np.empty(n, dtype=np.int32)
+    lib().orc_hsml_iter(d, n, k, dim, float(error_norm), int(max_iter), h, it)
+    return (h, it) if return_iters else h
+
+
+def kernel(q):
+    q = np.asarray(q, dtype=np.float64)
+    return np.where(q <= 0.5, 1 - 6 * q**2 + 6 * q**3, np.where(q <= 1.0, 2 * (1 - q) ** 3, 0.0))
+
+
+def num_derivs(dim, order):
+    return dim if order == 1 else {1: 2, 2: 5, 3: 9}[dim]
+
+
+def gradient_weights(pos, ngb, kernel_radius, indices, boxsize=None, weighted=True, order=1,
+                     return_status=False):
+    pos = _pos2d(pos)
+    n, dim = pos.shape
+    ngb = np.ascontiguousarray(ngb, dtype=np.int64)
+    nq, k = ngb.shape
+    h = np.ascontiguousarray(kernel_radius, dtype=np.float64)
+    idx = np.ascontiguousarray(indices, dtype=np.int64)
+    nd = num_derivs(dim, order)
+    out = np.empty((nq, nd, k), dtype=np.float64)
+    st = np.zeros(nq, dtype=np.int32)
+    lib().orc_gradient_weights(pos, n, dim, ngb, nq, k, h, idx,
+                               float(boxsize) if boxsize else -1.0, int(bool(weighted)),
+                               int(order), out, st)
+    return (out, st) if return_status else out
+
+
+def apply_weights(G, ngb, indices, f, row0=0, nrows=None):
+    """out[i, ..., a] = sum_j G[i,row0+a,j] (f[ngb[i,j]] - f[indices[i]])."""
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    nq, nd, k = G.shape
+    nrows = nd - row0 if nrows is None else nrows
+    f = np.asarray(f, dtype=np.float64)
+    ngb = np.asarray(ngb)
+    if ngb.size and (ngb.max() >= f.shape[0] or np.max(indices) >= f.shape[0]):
+        # np.take / fancy indexing in the reference raise IndexError here
+        raise IndexError("neighbour index out of range for f")
+    tail = f.shape[1:]
+    f2 = np.ascontiguousarray(f.reshape(f.shape[0], -1))
+    K = f2.shape[1]
+    out = np.empty((nq, K, nrows), dtype=np.float64)
+    lib().orc_apply_weights(G, nq, nd, k, int(row0), int(nrows),
+                            np.ascontiguousarray(ngb, dtype=np.int64),
+                            np.ascontiguousarray(indices, dtype=np.int64), f2, K, out)
+    return out.reshape((nq,) + tail + (nrows,))
+
+
+class OracleMeshoid:
+    """FP64 restatement of the reference ``Meshoid`` hot-path state machine
+    (``src/meshoid/Meshoid.py:15-170,237-331``), including its quirks:
+    ``density`` multiplies ``self.m[self.particle_mask]`` (Meshoid.py:169) and
+    ``Curl`` returns (dvy/dz - dvz/dy, dvx/dz - dvz/dx, dvx/dy - dvy/dx)
+    (Meshoid.py:311-315)."""
+
+    def __init__(self, pos, m=None, kernel_radius=None, des_ngb=None, boxsize=None,
+                 particle_mask=None):
+        pos = np.asarray(pos)
+        if pos.ndim == 1:
+            pos = pos[:, None]
+        self.N, self.dim = pos.shape
+        if particle_mask is None:
+            self.particle_mask = np.arange(self.N)
+        elif np.array(particle_mask).dtype == np.dtype("bool"):
+            self.particle_mask = np.arange(len(particle_mask))[particle_mask]
+        else:
+            self.particle_mask = np.asarray(particle_mask)
+        self.Nmask = len(self.particle_mask)
+        self.des_ngb = des_ngb if des_ngb is not None else {1: 4, 2: 20, 3: 32}[self.dim]
+        self.volnorm = {1: 2.0, 2: np.pi, 3: 4 * np.pi / 3}[self.dim]
+        self.boxsize = boxsize
+        self.pos = pos
+        if m is None:
+            m = np.repeat(1.0, self.N)
+        self.m = np.asarray(m)[self.particle_mask]
+        self.ngb = None
+        self.kernel_radius = kernel_radius
+        self.dweights = self.d2weights = self.dweights_3rdorder = None
+        if kernel_radius is None:
+            self.TreeUpdate()
+        else:
+            self.vol = self.volnorm * self.kernel_radius**self.dim / self.des_ngb
+            self.density = self.m / self.vol
+
+    def TreeUpdate(self):
+        self.ngbdist, self.ngb = knn(self.pos, self.des_ngb, self.boxsize,
+                                     queries=np.asarray(self.pos)[self.particle_mask])
+        self.kernel_radius = hsml_iter(self.ngbdist, error_norm=1e-13, dim=self.dim)
+        self.density = self.des_ngb * self.m[self.particle_mask] / (
+            self.volnorm * self.kernel_radius**self.dim)
+        self.vol = self.m[self.particle_mask] / self.density
+
+    def ComputeDWeights(self, order=1, weighted=True):
+        w = gradient_weights(self.pos, self.ngb, self.kernel_radius, self.particle_mask,
+                             boxsize=self.boxsize, weighted=weighted, order=order)
+        if order == 1:
+            self.dweights = w
+        else:
+            self.d2weights, self.dweights_3rdorder = w[:, self.dim:, :], w[:, :self.dim, :]
+
+    def D(self, f, order=1, weighted=True):
+        if self.ngb is None:
+            self.TreeUpdate()
+        if order == 1:
+            if self.dweights is None:
+                self.ComputeDWeights(1, weighted)
+            G = self.dweights
+        else:
+            if self.dweights_3rdorder is None:
+                self.ComputeDWeights(2, weighted)
+            G = self.dweights_3rdorder
+        return apply_weights(G, self.ngb, self.particle_mask, f)
+
+    def D2(self, f, weighted=True):
+        if self.ngb is None:
+            self.TreeUpdate()
+        if self.d2weights is None:
+            self.ComputeDWeights(2, weighted)
+        return apply_weights(self.d2weights, self.ngb, self.particle_mask, f)
+
+    def Curl(self, v):
+        dv = self.D(v[self.particle_mask])
+        return np.c_[dv[:, 1, 2] - dv[:, 2, 1], dv[:, 0, 2] - dv[:, 2, 0],
+                     dv[:, 0, 1] - dv[:, 1, 0]]
+
+    def Div(self, v):
+        dv = self.D(v[self.particle_mask])
+        return dv[:, 0, 0] + dv[:, 1, 1] + dv[:, 2, 2]
