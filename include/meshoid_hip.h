@@ -1,0 +1,152 @@
+/*
+ * meshoid_hip.h -- C ABI of libmeshoid_hip.so, the MI355X (gfx950) backend of
+ * meshoid's neighbour-weighted operator path.
+ *
+ * All array arguments are DEVICE pointers unless stated otherwise; sizes are
+ * element counts.  Calls are ordered on the context's stream; functions that
+ * must read a result on the host (index build: bounding box / range checks)
+ * synchronise that stream themselves.  Every function returns MHIP_OK or a
+ * negative MHIP_ERR_* code; mhip_last_error() gives the message.  Contexts are
+ * not thread-safe: use one per host thread / stream.
+ *
+ * The reference (meshoid 1.46, pure Python + Numba) has no C interface; each
+ * entry point below names the reference function or call it replaces.
+ */
+#ifndef MESHOID_HIP_H
+#define MESHOID_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHIP_OK 0
+#define MHIP_ERR_VALUE -1       /* invalid argument / data      -> ValueError   */
+#define MHIP_ERR_HIP -2         /* HIP runtime failure          -> RuntimeError */
+#define MHIP_ERR_SINGULAR -3    /* singular moment matrix       -> LinAlgError  */
+#define MHIP_ERR_UNSUPPORTED -4 /* outside the implemented range -> NotImplementedError */
+#define MHIP_ERR_KEY -5         /* dim not in {1,2,3}           -> KeyError     */
+
+/* per-row status bits (status arrays) */
+#define MHIP_ST_AMBIGUOUS 1      /* fp32 could not order k-th/(k+1)-th: fp64 resolved */
+#define MHIP_ST_BAND_OVERFLOW 2  /* more fp64 ties at the k-th distance than held     */
+#define MHIP_ST_HSML_NOCONV 4    /* smoothing-length iteration hit its cap            */
+#define MHIP_ST_SINGULAR 8       /* moment matrix exactly singular                    */
+#define MHIP_ST_COUNT 16         /* neighbour collection did not return k members     */
+#define MHIP_ST_SEED_MISS 32     /* learned seed rejected, bracket used               */
+#define MHIP_ST_RANKDEF 64       /* moment matrix rank-deficient by rounding: row NaN */
+
+/* mhip_knn_hsml flags */
+#define MHIP_KNN_KEEP_DIST 1 /* keep squared distances (needed by mhip_nlist_export) */
+
+typedef struct mhip_ctx mhip_ctx;
+typedef struct mhip_index mhip_index;
+typedef struct mhip_nlist mhip_nlist;
+
+/* library version, e.g. 10000 for 1.0.0 */
+int mhip_version(void);
+
+/* Context: device ordinal + stream (NULL = the default stream of that device). */
+int mhip_ctx_create(int device, void *stream, mhip_ctx **out);
+int mhip_ctx_destroy(mhip_ctx *ctx);
+int mhip_ctx_set_stream(mhip_ctx *ctx, void *stream);
+int mhip_ctx_synchronize(mhip_ctx *ctx);
+const char *mhip_last_error(const mhip_ctx *ctx);
+
+/*
+ * Spatial index over n particles.  Replaces BuildTree / cKDTree(pos, boxsize)
+ * (src/meshoid/Meshoid.py:142-145).  pos: (n, dim) row-major, float32
+ * (pos_f64 = 0) or float64 (pos_f64 = 1); the buffer must stay alive while the
+ * index is used (float64 neighbour-boundary resolution reads it).  boxsize > 0
+ * makes the space periodic on [0, boxsize)^dim; data outside raises
+ * MHIP_ERR_VALUE as cKDTree raises ValueError.  Rebuilding an index reuses its
+ * buffers when they are large enough.
+ */
+int mhip_index_create(mhip_index **out);
+int mhip_index_build(mhip_ctx *ctx, mhip_index *ix, const void *pos, int pos_f64, int64_t n,
+                     int dim, double boxsize);
+int mhip_index_destroy(mhip_index *ix);
+/* host-side copy of n, dim, boxsize and the bounding box (lo xyz, hi xyz) */
+int mhip_index_info(const mhip_index *ix, int64_t *n, int *dim, double *boxsize, float *bbox6);
+
+/*
+ * Exact kNN (k = des_ngb, self included) + smoothing length + density for the
+ * query rows.  Replaces TreeUpdate (src/meshoid/Meshoid.py:147-170):
+ * tree.query(pos[mask], des_ngb) (:157), HsmlIter(ngbdist, error_norm=1e-13,
+ * dim) (:165, src/meshoid/kernel_density.py:14-57) and
+ * density = des_ngb*m/(volnorm*h^dim), vol = m/density (:169-170).
+ *   qidx     int64[nq] particle index of each row (particle_mask); NULL = all
+ *            particles in order (nq is then ignored and taken as n)
+ *   m_rows   float64[nq] mass of each row; NULL = 1
+ *   seed_h   float32[nq] initial smoothing-length guess per row (learned seed);
+ *            NULL = none.  A guess outside the bracket is ignored.
+ *   h_out, rho_out, vol_out   float32[nq], any may be NULL
+ *   status_out int32[nq] MHIP_ST_* bits, may be NULL
+ *   nl       neighbour list filled for later mhip_nlist_export / mhip_apply_D
+ * Supports 2 <= k <= 128 and k <= n.
+ */
+int mhip_nlist_create(mhip_nlist **out);
+int mhip_nlist_destroy(mhip_nlist *nl);
+int mhip_knn_hsml(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
+                  const double *m_rows, const float *seed_h, float *h_out, float *rho_out,
+                  float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl);
+
+/*
+ * Neighbour rows as cKDTree.query returns them (src/meshoid/Meshoid.py:157):
+ * ngb_out int32[nq*k] original particle indices, dist_out float32[nq*k]
+ * distances, each row ascending by distance (ties by index).  Either output
+ * may be NULL.  Needs a list built with MHIP_KNN_KEEP_DIST.
+ */
+int mhip_nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out);
+
+/*
+ * Least-squares derivative operators over the list's rows.  Replaces
+ * Meshoid.D (src/meshoid/Meshoid.py:237-268), Meshoid.D2 (:270-295) with
+ * ComputeDWeights (:110-140) and gradient_weights
+ * (src/meshoid/derivatives.py:117-138), fused: the weights are never stored.
+ *   f        (nf_rows, K) row-major, float32 (f_f64 = 0) or float64; row r is
+ *            the value at original particle r (the reference's np.take(f, ngb))
+ *   order    1: out_d1 = gradient (nq, K, dim)
+ *            2: quadratic fit; out_d1 = first derivatives (nq, K, dim) as
+ *               D(f, order=2), out_d2 = second derivatives (nq, K, nd2) with
+ *               nd2 = 6 in 3D (xx,yy,zz,xy,yz,zx), 3 in 2D, 1 in 1D, as D2(f)
+ *   weighted kernel-weighted (1) or unweighted (0) least squares
+ *   status   int32[nq] per row (MHIP_ST_SINGULAR / MHIP_ST_RANKDEF, as for
+ *            mhip_gradient_weights), may be NULL
+ * Outputs are float32; either may be NULL.
+ */
+int mhip_apply_D(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows,
+                 int K, int order, int weighted, float *out_d1, float *out_d2, int32_t *status);
+
+/*
+ * gradient_weights(pos, ngb, kernel_radius, indices, boxsize, weighted, order)
+ * (src/meshoid/derivatives.py:117-138) on arbitrary row-major neighbour lists:
+ * pos (n, dim) float32/float64, ngb int32[nq*k], h float64[nq],
+ * indices int64[nq]; G float64[nq * nd * k] with nd = dim (order 1) or
+ * {1:2, 2:5, 3:9}[dim] (order 2).  status int32[nq] may be NULL:
+ * MHIP_ST_SINGULAR when the moment matrix has an exactly zero diagonal entry
+ * (np.linalg.inv raises there; the Python layer raises LinAlgError),
+ * MHIP_ST_RANKDEF when the LDL^T meets a zero pivot (the row is NaN; LU
+ * returns unbounded values there).
+ */
+int mhip_gradient_weights(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
+                          const int32_t *ngb, int64_t nq, int k, const double *h,
+                          const int64_t *indices, double boxsize, int weighted, int order,
+                          double *G, int32_t *status);
+
+/*
+ * HsmlIter(neighbor_dists, dim, error_norm) (src/meshoid/kernel_density.py:14-57)
+ * on a given (n, k) distance array (float32/float64, rows ascending):
+ * h float64[n]; status int32[n] (MHIP_ST_HSML_NOCONV) may be NULL.
+ */
+int mhip_hsml_iter(mhip_ctx *ctx, const void *dists, int dists_f64, int64_t n, int k, int dim,
+                   double error_norm, double *h, int32_t *status);
+
+/* Kernel(q) ufunc (src/meshoid/kernel_density.py:82-96), elementwise, same dtype */
+int mhip_kernel(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MESHOID_HIP_H */

@@ -1,0 +1,85 @@
+"""ctypes binding of libmeshoid_hip.so (C ABI in include/meshoid_hip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded,
+importing this module raises ImportError (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("MESHOID_HIP_LIB", os.path.join(HERE, "libmeshoid_hip.so"))
+
+ERR_VALUE, ERR_HIP, ERR_SINGULAR, ERR_UNSUPPORTED, ERR_KEY = -1, -2, -3, -4, -5
+ST_AMBIGUOUS, ST_BAND_OVERFLOW, ST_HSML_NOCONV, ST_SINGULAR, ST_COUNT, ST_SEED_MISS = (
+    1, 2, 4, 8, 16, 32)
+ST_RANKDEF = 64
+KNN_KEEP_DIST = 1
+
+EXPORTED = [
+    "mhip_version", "mhip_ctx_create", "mhip_ctx_destroy", "mhip_ctx_set_stream",
+    "mhip_ctx_synchronize", "mhip_last_error", "mhip_index_create", "mhip_index_build",
+    "mhip_index_destroy", "mhip_index_info", "mhip_nlist_create", "mhip_nlist_destroy",
+    "mhip_knn_hsml", "mhip_nlist_export", "mhip_apply_D", "mhip_gradient_weights",
+    "mhip_hsml_iter", "mhip_kernel",
+]
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} not found: build it with `python -m meshoid_amd.build` "
+        "(hipcc, gfx950). meshoid_amd has no CPU fallback.")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_pp = ctypes.POINTER(ctypes.c_void_p)
+
+_SIGS = {
+    "mhip_version": ([], _i),
+    "mhip_ctx_create": ([_i, _p, _pp], _i),
+    "mhip_ctx_destroy": ([_p], _i),
+    "mhip_ctx_set_stream": ([_p, _p], _i),
+    "mhip_ctx_synchronize": ([_p], _i),
+    "mhip_last_error": ([_p], ctypes.c_char_p),
+    "mhip_index_create": ([_pp], _i),
+    "mhip_index_build": ([_p, _p, _p, _i, _i64, _i, _d], _i),
+    "mhip_index_destroy": ([_p], _i),
+    "mhip_index_info": ([_p, _p, _p, _p, _p], _i),
+    "mhip_nlist_create": ([_pp], _i),
+    "mhip_nlist_destroy": ([_p], _i),
+    "mhip_knn_hsml": ([_p, _p, _p, _i64, _i, _p, _p, _p, _p, _p, _p, _i, _p], _i),
+    "mhip_nlist_export": ([_p, _p, _p, _p], _i),
+    "mhip_apply_D": ([_p, _p, _p, _i, _i64, _i, _i, _i, _p, _p, _p], _i),
+    "mhip_gradient_weights": ([_p, _p, _i, _i64, _i, _p, _i64, _i, _p, _p, _d, _i, _i, _p, _p],
+                              _i),
+    "mhip_hsml_iter": ([_p, _p, _i, _i64, _i, _i, _d, _p, _p], _i),
+    "mhip_kernel": ([_p, _p, _i, _i64, _p], _i),
+}
+for _name, (_args, _res) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.argtypes = _args
+    _fn.restype = _res
+
+
+def raise_for(rc: int, ctx=None, what: str = ""):
+    """Map a MHIP_ERR_* code to the exception the reference raises."""
+    if rc == 0:
+        return
+    msg = lib.mhip_last_error(ctx).decode() if ctx else what
+    if what and msg != what:
+        msg = f"{what}: {msg}"
+    if rc == ERR_VALUE:
+        raise ValueError(msg)
+    if rc == ERR_SINGULAR:
+        raise np.linalg.LinAlgError(msg)
+    if rc == ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if rc == ERR_KEY:
+        raise KeyError(msg)
+    raise RuntimeError(msg)

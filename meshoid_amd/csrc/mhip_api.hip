@@ -1,0 +1,194 @@
+// C ABI of libmeshoid_hip.so (declared in include/meshoid_hip.h): argument
+// checks, context/scratch management and error plumbing around the kernels.
+#include <stdio.h>
+
+#include <new>
+
+#include "../../include/meshoid_hip.h"
+#include "mhip_internal.hpp"
+
+namespace mhip {
+
+int fail(mhip_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+int check_hip(mhip_ctx *ctx, hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    return fail(ctx, MHIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// grow-only per-context device scratch; callers use it within one call, on
+// the context's stream, so a grow waits for earlier work before freeing
+void *scratch(mhip_ctx *ctx, size_t bytes, int *rc) {
+    *rc = 0;
+    if (bytes <= ctx->scratch_bytes) return ctx->scratch;
+    if (ctx->scratch) {
+        hipStreamSynchronize(ctx->stream);
+        hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&ctx->scratch, want);
+    if (e != hipSuccess) {
+        *rc = check_hip(ctx, e, "hipMalloc (scratch)");
+        ctx->scratch = nullptr;
+        return nullptr;
+    }
+    ctx->scratch_bytes = want;
+    return ctx->scratch;
+}
+
+}  // namespace mhip
+
+using namespace mhip;
+
+#define MHIP_NEED(cond, msg) \
+    do {                      \
+        if (!(cond)) return fail(ctx, MHIP_ERR_VALUE, msg); \
+    } while (0)
+
+extern "C" {
+
+int mhip_version(void) { return 10000; }
+
+int mhip_ctx_create(int device, void *stream, mhip_ctx **out) {
+    if (!out) return MHIP_ERR_VALUE;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MHIP_ERR_HIP;
+    if (device < 0 || device >= ndev) return MHIP_ERR_VALUE;
+    if (hipSetDevice(device) != hipSuccess) return MHIP_ERR_HIP;
+    mhip_ctx *c = new (std::nothrow) mhip_ctx();
+    if (!c) return MHIP_ERR_HIP;
+    c->device = device;
+    c->stream = (hipStream_t)stream;
+    *out = c;
+    return MHIP_OK;
+}
+
+int mhip_ctx_destroy(mhip_ctx *ctx) {
+    if (!ctx) return MHIP_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) hipFree(ctx->scratch);
+    delete ctx;
+    return MHIP_OK;
+}
+
+int mhip_ctx_set_stream(mhip_ctx *ctx, void *stream) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    ctx->stream = (hipStream_t)stream;
+    return MHIP_OK;
+}
+
+int mhip_ctx_synchronize(mhip_ctx *ctx) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "stream synchronize");
+}
+
+const char *mhip_last_error(const mhip_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+int mhip_index_create(mhip_index **out) {
+    if (!out) return MHIP_ERR_VALUE;
+    *out = new (std::nothrow) mhip_index();
+    return *out ? MHIP_OK : MHIP_ERR_HIP;
+}
+
+int mhip_index_destroy(mhip_index *ix) {
+    if (!ix) return MHIP_OK;
+    void *arrs[] = {ix->sx, ix->sy, ix->sz, ix->lx, ix->ly, ix->lz, ix->perm, ix->iperm, ix->nodes};
+    for (void *p : arrs)
+        if (p) hipFree(p);
+    delete ix;
+    return MHIP_OK;
+}
+
+int mhip_index_build(mhip_ctx *ctx, mhip_index *ix, const void *pos, int pos_f64, int64_t n, int dim,
+                     double boxsize) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(ix && pos, "null index or positions");
+    hipSetDevice(ctx->device);
+    return index_build(ctx, pos, pos_f64 ? 1 : 0, n, dim, boxsize > 0 ? boxsize : -1.0, ix);
+}
+
+int mhip_index_info(const mhip_index *ix, int64_t *n, int *dim, double *boxsize, float *bbox6) {
+    if (!ix) return MHIP_ERR_VALUE;
+    if (n) *n = ix->n;
+    if (dim) *dim = ix->dim;
+    if (boxsize) *boxsize = ix->box;
+    if (bbox6)
+        for (int c = 0; c < 6; c++) bbox6[c] = ix->bbox[c];
+    return MHIP_OK;
+}
+
+int mhip_nlist_create(mhip_nlist **out) {
+    if (!out) return MHIP_ERR_VALUE;
+    *out = new (std::nothrow) mhip_nlist();
+    return *out ? MHIP_OK : MHIP_ERR_HIP;
+}
+
+int mhip_nlist_destroy(mhip_nlist *nl) {
+    if (!nl) return MHIP_OK;
+    void *arrs[] = {nl->qsorted, nl->qrow, nl->idx, nl->d2, nl->tau, nl->nlt, nl->h, nl->status};
+    for (void *p : arrs)
+        if (p) hipFree(p);
+    delete nl;
+    return MHIP_OK;
+}
+
+int mhip_knn_hsml(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
+                  const double *m_rows, const float *seed_h, float *h_out, float *rho_out, float *vol_out,
+                  int32_t *status_out, int flags, mhip_nlist *nl) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(ix && nl, "null index or neighbour list");
+    MHIP_NEED(ix->n > 0, "index has not been built");
+    hipSetDevice(ctx->device);
+    return knn_run(ctx, ix, qidx, nq, k, m_rows, seed_h, h_out, rho_out, vol_out, status_out, flags, nl);
+}
+
+int mhip_nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(nl, "null neighbour list");
+    hipSetDevice(ctx->device);
+    return nlist_export(ctx, nl, ngb_out, dist_out);
+}
+
+int mhip_apply_D(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows, int K,
+                 int order, int weighted, float *out_d1, float *out_d2, int32_t *status) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(nl && f, "null neighbour list or field");
+    MHIP_NEED(nf_rows >= 0, "negative field length");
+    hipSetDevice(ctx->device);
+    return apply_ops(ctx, nl, f, f_f64 ? 1 : 0, nf_rows, K, order, weighted ? 1 : 0, out_d1, out_d2,
+                     status);
+}
+
+int mhip_gradient_weights(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim, const int32_t *ngb,
+                          int64_t nq, int k, const double *h, const int64_t *indices, double boxsize,
+                          int weighted, int order, double *G, int32_t *status) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(pos && ngb && h && indices && G, "null argument");
+    hipSetDevice(ctx->device);
+    return weights_rows(ctx, pos, pos_f64 ? 1 : 0, n, dim, ngb, nq, k, h, indices,
+                        boxsize > 0 ? boxsize : -1.0, weighted ? 1 : 0, order, G, status);
+}
+
+int mhip_hsml_iter(mhip_ctx *ctx, const void *dists, int dists_f64, int64_t n, int k, int dim,
+                   double error_norm, double *h, int32_t *status) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(dists && h, "null argument");
+    hipSetDevice(ctx->device);
+    return hsml_dists(ctx, dists, dists_f64 ? 1 : 0, n, k, dim, error_norm, h, status);
+}
+
+int mhip_kernel(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(q && out, "null argument");
+    hipSetDevice(ctx->device);
+    return kernel_eval(ctx, q, f64 ? 1 : 0, n, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// Shared device helpers and internal host structures of libmeshoid_hip.
+//
+// Conventions used by every kernel in this library:
+//   * particles are reordered once into Morton order ("sorted space"); positions
+//     live there as float32 structure-of-arrays sx/sy/sz padded to a multiple of
+//     LEAF with NaN coordinates (every distance comparison against a padding
+//     slot is false, also after a periodic wrap).  Float64 inputs keep a float32 residual
+//     (lx/ly/lz) so that x = hi + lo to ~2^-48.
+//   * dim < 3 is handled by zero coordinates in the unused axes.
+//   * one wavefront (64 lanes) serves 64 consecutive queries in sorted order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#define MHIP_DEV __device__ __forceinline__
+
+namespace mhip {
+
+constexpr int WAVE = 64;
+constexpr int LEAF = 16;    // particles per leaf of the implicit tree
+constexpr int FANOUT = 8;   // children per internal node
+constexpr int MAX_LEVELS = 16;
+
+// status bits written per query row
+enum : int32_t {
+    ST_AMBIGUOUS = 1,      // k-th / (k+1)-th distances within fp32 rounding: resolved in fp64
+    ST_BAND_OVERFLOW = 2,  // more fp64 ties at the k-th distance than the resolver holds
+    ST_HSML_NOCONV = 4,    // smoothing-length iteration hit its cap
+    ST_SINGULAR = 8,       // exactly singular moment matrix (zero pivot)
+    ST_COUNT = 16,         // neighbour collection did not return exactly k members
+    ST_SEED_MISS = 32,     // learned seed rejected, bracket used instead
+    ST_RANKDEF = 64,       // moment matrix rank-deficient at rounding level: row is NaN
+};
+
+// ---------------------------------------------------------------------------
+// cubic-spline kernel W(q) (unnormalised), src/meshoid/kernel_density.py:82-96
+//   1 - 6q^2 + 6q^3   q <= 1/2
+//   2 (1-q)^3         1/2 < q <= 1
+//   0                 q > 1
+// ---------------------------------------------------------------------------
+template <typename T>
+MHIP_DEV T kernel_w(T q) {
+    T a = T(1) - q;
+    T inner = T(1) + T(6) * q * q * (q - T(1));
+    T outer = T(2) * a * a * a;
+    T w = q <= T(0.5) ? inner : outer;
+    return q <= T(1) ? w : T(0);
+}
+
+// W and dW/dq together
+template <typename T>
+MHIP_DEV void kernel_w_dw(T q, T &w, T &dw) {
+    T a = T(1) - q;
+    bool in = q <= T(0.5);
+    T wi = T(1) + T(6) * q * q * (q - T(1));
+    T dwi = q * (T(18) * q - T(12));
+    T wo = T(2) * a * a * a;
+    T dwo = T(-6) * a * a;
+    w = in ? wi : wo;
+    dw = in ? dwi : dwo;
+    if (!(q <= T(1))) {
+        w = T(0);
+        dw = T(0);
+    }
+}
+
+// norm_d of kernel_density.py:26-31 and the bracket factor of :35
+MHIP_DEV double hsml_norm(int dim) { return dim == 3 ? 32.0 / 3 : (dim == 2 ? 40.0 / 7 : 8.0 / 3); }
+MHIP_DEV double hsml_bound_coeff(int dim) {
+    return dim == 3 ? 1.5638742363463305 : (dim == 2 ? 1.7984058958344455 : 2.33839974837265);
+}
+
+// nearest_image of src/meshoid/derivatives.py:8-14 (box > 0)
+template <typename T>
+MHIP_DEV T wrap(T d, T box) {
+    T ad = d < T(0) ? -d : d;
+    if (ad > box * T(0.5)) return d > T(0) ? -(box - ad) : (box - ad);
+    return d;
+}
+
+MHIP_DEV float med3(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
+
+MHIP_DEV uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// XCD-aware block remap: consecutive logical blocks land on the same XCD
+// (the hardware dispatches physical block b to XCD b % 8), so neighbouring
+// query groups share that XCD's L2.
+MHIP_DEV uint32_t xcd_remap(uint32_t b, uint32_t nblocks) {
+    constexpr uint32_t NX = 8;
+    uint32_t per = nblocks / NX, rem = nblocks % NX;
+    uint32_t x = b % NX, i = b / NX;
+    // XCDs 0..rem-1 own per+1 logical blocks, the rest own per
+    uint32_t start = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    return start + i;
+}
+
+}  // namespace mhip
+
+// ---------------------------------------------------------------------------
+// host-side objects behind the opaque C handles
+// ---------------------------------------------------------------------------
+struct mhip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    void *hscratch = nullptr;  // small pinned host buffer for readbacks
+};
+
+struct mhip_index {
+    int64_t n = 0, npad = 0, cap = 0;
+    int dim = 3;
+    double box = -1.0;  // > 0: periodic in [0, box)^dim
+    int pos_f64 = 0;
+    // sorted space
+    float *sx = nullptr, *sy = nullptr, *sz = nullptr;  // hi parts, npad entries
+    float *lx = nullptr, *ly = nullptr, *lz = nullptr;  // lo parts (f64 inputs only)
+    uint32_t *perm = nullptr;                           // sorted -> original
+    uint32_t *iperm = nullptr;                          // original -> sorted
+    const void *pos_in = nullptr;                       // caller's positions (for fp64 resolution)
+    // implicit tree: level 0 = leaves; node boxes as 8 floats (lo xyz, hi xyz, pad)
+    int nlev = 0;
+    int64_t lev_n[mhip::MAX_LEVELS] = {};
+    int64_t lev_off[mhip::MAX_LEVELS] = {};
+    float *nodes = nullptr;
+    int64_t node_cap = 0;
+    float bbox[6] = {};
+};
+
+struct mhip_nlist {
+    int64_t nq = 0, cap = 0;
+    int k = 0;
+    uint32_t *qsorted = nullptr;  // sorted-space particle index of each query slot
+    uint32_t *qrow = nullptr;     // output row of each query slot
+    uint32_t *idx = nullptr;      // column-major [k][nq] sorted-space neighbour indices
+    float *d2 = nullptr;          // column-major [k][nq] squared distances
+    float *tau = nullptr;         // k-th squared distance per slot
+    int32_t *nlt = nullptr;       // count strictly below tau per slot
+    float *h = nullptr;           // smoothing length per slot
+    int32_t *status = nullptr;    // per slot
+    int masked = 0;               // qsorted/qrow valid (query rows are a subset/reordering)
+    int has_d2 = 0;               // d2 filled (needed for export)
+    const mhip_index *index = nullptr;
+};

@@ -1,0 +1,384 @@
+// Least-squares derivative operators on the GPU.
+//
+// Replaces, for the hot path,
+//   gradient_weights(...)                       src/meshoid/derivatives.py:117-138
+//     kernel_dx_and_weights                     src/meshoid/derivatives.py:26-52
+//     polyfit_leastsq_matrices                  src/meshoid/derivatives.py:55-103
+//   np.einsum("ikj,ij...->i...k", G, df)        src/meshoid/Meshoid.py:258,268,291,295
+//
+// The fused operator never materialises G = E^-1 [w a_j]: since
+//   D f_i = sum_j G_ij (f_j - f_i) = E_i^-1 sum_j w_ij a_ij (f_j - f_i),
+// each thread accumulates the moment matrix E (float64, from float32 pair
+// offsets relative to the host particle) and the right-hand side of every
+// field, then solves with an unpivoted float64 LDL^T (E is symmetric positive
+// semi-definite; an exactly zero or NaN pivot is reported as singular, as
+// np.linalg.inv raises on an exactly zero pivot).
+#include "mhip_internal.hpp"
+#include "mhip_linalg.hpp"
+
+namespace mhip {
+
+// ---------------------------------------------------------------------------
+// fused operator over the neighbour list
+// ---------------------------------------------------------------------------
+struct ApplyArgs {
+    const float *sx, *sy, *sz, *lx, *ly, *lz;
+    float box, inv_box;
+    int per, p64;
+    const uint32_t *nl_idx;
+    const uint32_t *qsorted, *qrow, *perm;
+    const float *h;
+    int64_t nq, npad;
+    int k;
+    const void *fs;  // [K][npad] sorted-space fields
+    int K;
+    int weighted;
+    float *out1, *out2;  // [row][K][DIM], [row][K][ND-DIM]
+    int32_t *status;     // per row, may be null
+};
+
+
+template <int DIM, int ORDER, typename FT, int KC>
+__global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
+    constexpr int ND = NDer<DIM, ORDER>::ND, NE = NDer<DIM, ORDER>::NE;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t s = (int64_t)b * blockDim.x + threadIdx.x;
+    if (s >= a.nq) return;
+    const uint32_t qp = a.qsorted ? a.qsorted[s] : (uint32_t)s;
+    const uint32_t row = a.qrow ? a.qrow[s] : a.perm[s];
+    // host particle in float64: hi + lo reproduces float64 inputs to 2^-48
+    // and float32 inputs exactly, so the pair offsets below are exact for
+    // float32 positions
+    const double qx = (double)a.sx[qp] + (a.p64 ? (double)a.lx[qp] : 0.0);
+    const double qy = (double)a.sy[qp] + (a.p64 ? (double)a.ly[qp] : 0.0);
+    const double qz = (double)a.sz[qp] + (a.p64 ? (double)a.lz[qp] : 0.0);
+    const double box = (double)a.box;
+    const float hh = a.h[s];
+    const double ih = hh > 0.f ? 1.0 / (double)hh : (double)INFINITY;
+    const FT *fs = (const FT *)a.fs;
+    int32_t st = 0;
+    for (int f0 = 0; f0 < a.K; f0 += KC) {
+        double E[NE];
+        double B[KC][ND];
+        FT fc[KC];
+#pragma unroll
+        for (int i = 0; i < NE; i++) E[i] = 0.0;
+#pragma unroll
+        for (int f = 0; f < KC; f++) {
+#pragma unroll
+            for (int i = 0; i < ND; i++) B[f][i] = 0.0;
+            fc[f] = f0 + f < a.K ? fs[(int64_t)(f0 + f) * a.npad + qp] : FT(0);
+        }
+        for (int j = 0; j < a.k; j++) {
+            const uint32_t p = a.nl_idx[(int64_t)j * a.nq + s];
+            double d[3] = {(double)a.sx[p] - qx, (double)a.sy[p] - qy, (double)a.sz[p] - qz};
+            if (a.p64) {
+                d[0] += (double)a.lx[p];
+                d[1] += (double)a.ly[p];
+                d[2] += (double)a.lz[p];
+            }
+            if (a.per) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) d[c] = wrap<double>(d[c], box);
+            }
+            double w = 1.0;
+            if (a.weighted) w = kernel_w<double>(sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
+            double av[ND];
+            basis<DIM, ORDER, double>(d, av);
+            double wa[ND];
+#pragma unroll
+            for (int i = 0; i < ND; i++) wa[i] = w * av[i];
+#pragma unroll
+            for (int i = 0; i < ND; i++) {
+#pragma unroll
+                for (int jj = 0; jj <= i; jj++) E[tri(i, jj)] = fma(wa[i], av[jj], E[tri(i, jj)]);
+            }
+#pragma unroll
+            for (int f = 0; f < KC; f++) {
+                if (f0 + f < a.K) {
+                    const double df = (double)fs[(int64_t)(f0 + f) * a.npad + p] - (double)fc[f];
+#pragma unroll
+                    for (int i = 0; i < ND; i++) B[f][i] = fma(wa[i], df, B[f][i]);
+                }
+            }
+        }
+        if (zero_diagonal<ND>(E)) st |= ST_SINGULAR;
+        const bool ok = ldlt<ND>(E);
+        if (!ok) st |= ST_RANKDEF;
+#pragma unroll
+        for (int f = 0; f < KC; f++) {
+            if (f0 + f >= a.K) continue;
+            ldlt_solve<ND>(E, B[f]);
+            if (!ok) {
+#pragma unroll
+                for (int i = 0; i < ND; i++) B[f][i] = (double)NAN;
+            }
+            if (a.out1) {
+                float *o = a.out1 + ((int64_t)row * a.K + f0 + f) * DIM;
+#pragma unroll
+                for (int i = 0; i < DIM; i++) o[i] = (float)B[f][i];
+            }
+            if (ORDER == 2 && a.out2) {
+                float *o = a.out2 + ((int64_t)row * a.K + f0 + f) * (ND - DIM);
+#pragma unroll
+                for (int i = DIM; i < ND; i++) o[i - DIM] = (float)B[f][i];
+            }
+        }
+    }
+    if (a.status) a.status[row] = st;
+}
+
+template <typename FT>
+__global__ __launch_bounds__(256) void k_permute_fields(const FT *__restrict__ f, int64_t nf_rows, int K,
+                                                       const uint32_t *__restrict__ perm, int64_t n,
+                                                       int64_t npad, FT *__restrict__ fs) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= npad) return;
+    const uint32_t o = p < n ? perm[p] : 0xffffffffu;
+    for (int c = 0; c < K; c++) {
+        FT v = (o < nf_rows) ? f[(int64_t)o * K + c] : (FT)NAN;
+        fs[(int64_t)c * npad + p] = v;
+    }
+}
+
+template <int DIM, int ORDER, typename FT>
+static void launch_apply(const ApplyArgs &a, dim3 g, hipStream_t s) {
+    if (ORDER == 1)
+        hipLaunchKernelGGL((k_apply<DIM, ORDER, FT, 4>), g, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_apply<DIM, ORDER, FT, 2>), g, dim3(256), 0, s, a);
+}
+
+template <typename FT>
+static void dispatch_apply(const ApplyArgs &a, int dim, int order, dim3 g, hipStream_t s) {
+    if (order == 1) {
+        if (dim == 1) launch_apply<1, 1, FT>(a, g, s);
+        if (dim == 2) launch_apply<2, 1, FT>(a, g, s);
+        if (dim == 3) launch_apply<3, 1, FT>(a, g, s);
+    } else {
+        if (dim == 1) launch_apply<1, 2, FT>(a, g, s);
+        if (dim == 2) launch_apply<2, 2, FT>(a, g, s);
+        if (dim == 3) launch_apply<3, 2, FT>(a, g, s);
+    }
+}
+
+int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows, int K,
+              int order, int weighted, float *out1, float *out2, int32_t *status) {
+    const mhip_index *ix = nl->index;
+    if (!ix || nl->nq < 1) return fail(ctx, -1, "neighbour list is empty");
+    if (order != 1 && order != 2) return fail(ctx, -1, "order must be 1 or 2");
+    if (K < 1) return fail(ctx, -1, "no field components");
+    int rc = 0;
+    const size_t fb = f_f64 ? 8 : 4;
+    void *fs = scratch(ctx, (size_t)K * ix->npad * fb, &rc);
+    if (!fs) return rc;
+    const dim3 gp((unsigned)((ix->npad + 255) / 256));
+    if (f_f64)
+        hipLaunchKernelGGL(k_permute_fields<double>, gp, dim3(256), 0, ctx->stream, (const double *)f,
+                           nf_rows, K, ix->perm, ix->n, ix->npad, (double *)fs);
+    else
+        hipLaunchKernelGGL(k_permute_fields<float>, gp, dim3(256), 0, ctx->stream, (const float *)f,
+                           nf_rows, K, ix->perm, ix->n, ix->npad, (float *)fs);
+    ApplyArgs a;
+    a.sx = ix->sx;
+    a.sy = ix->sy;
+    a.sz = ix->sz;
+    a.lx = ix->lx;
+    a.ly = ix->ly;
+    a.lz = ix->lz;
+    a.per = ix->box > 0;
+    a.box = a.per ? (float)ix->box : 0.f;
+    a.inv_box = a.per ? (float)(1.0 / ix->box) : 0.f;
+    a.p64 = ix->pos_f64;
+    a.nl_idx = nl->idx;
+    a.qsorted = nl->masked ? nl->qsorted : nullptr;
+    a.qrow = nl->masked ? nl->qrow : nullptr;
+    a.perm = ix->perm;
+    a.h = nl->h;
+    a.nq = nl->nq;
+    a.npad = ix->npad;
+    a.k = nl->k;
+    a.fs = fs;
+    a.K = K;
+    a.weighted = weighted;
+    a.out1 = out1;
+    a.out2 = out2;
+    a.status = status;
+    const dim3 g((unsigned)((nl->nq + 255) / 256));
+    if (f_f64)
+        dispatch_apply<double>(a, ix->dim, order, g, ctx->stream);
+    else
+        dispatch_apply<float>(a, ix->dim, order, g, ctx->stream);
+    return check_hip(ctx, hipGetLastError(), "apply launch");
+}
+
+// ---------------------------------------------------------------------------
+// materialised weights G (nq, ND, k) float64 for arbitrary row-major neighbour
+// lists, gradient_weights (derivatives.py:117-138) signature
+// ---------------------------------------------------------------------------
+template <int DIM, int ORDER, typename PT>
+__global__ __launch_bounds__(128) void k_weights_rows(const PT *__restrict__ pos, const int32_t *__restrict__ ngb,
+                                                     int64_t nq, int k, const double *__restrict__ h,
+                                                     const int64_t *__restrict__ indices, double box,
+                                                     int weighted, double *__restrict__ G,
+                                                     int32_t *__restrict__ status) {
+    constexpr int ND = NDer<DIM, ORDER>::ND, NE = NDer<DIM, ORDER>::NE;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const int64_t c = indices[i];
+    double xc[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < DIM; a++) xc[a] = (double)pos[c * DIM + a];
+    const double ih = 1.0 / h[i];
+    double E[NE];
+#pragma unroll
+    for (int e = 0; e < NE; e++) E[e] = 0.0;
+    auto pair = [&](int j, double (&av)[ND], double &w) {
+        const int64_t nb = ngb[i * k + j];
+        double d[3] = {0, 0, 0};
+        double r2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < DIM; a++) {
+            double v = (double)pos[nb * DIM + a] - xc[a];
+            if (box > 0) v = wrap<double>(v, box);
+            d[a] = v;
+            r2 += v * v;
+        }
+        w = weighted ? kernel_w<double>(sqrt(r2) * ih) : 1.0;
+        basis<DIM, ORDER, double>(d, av);
+    };
+    for (int j = 0; j < k; j++) {
+        double av[ND], w;
+        pair(j, av, w);
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+#pragma unroll
+            for (int b2 = 0; b2 <= a; b2++) E[tri(a, b2)] = fma(w * av[a], av[b2], E[tri(a, b2)]);
+        }
+    }
+    const bool zero = zero_diagonal<ND>(E);
+    const bool ok = ldlt<ND>(E);
+    if (status) status[i] = (zero ? ST_SINGULAR : 0) | (ok ? 0 : ST_RANKDEF);
+    double *out = G + i * (int64_t)ND * k;
+    for (int j = 0; j < k; j++) {
+        double av[ND], w;
+        pair(j, av, w);
+#pragma unroll
+        for (int a = 0; a < ND; a++) av[a] *= w;
+        ldlt_solve<ND>(E, av);
+#pragma unroll
+        for (int a = 0; a < ND; a++) out[(int64_t)a * k + j] = ok ? av[a] : NAN;
+    }
+}
+
+template <typename PT>
+static void dispatch_weights(const PT *pos, int dim, int order, const int32_t *ngb, int64_t nq, int k,
+                             const double *h, const int64_t *indices, double box, int weighted,
+                             double *G, int32_t *status, hipStream_t s) {
+    const dim3 g((unsigned)((nq + 127) / 128)), b(128);
+#define MHIP_W(D, O) \
+    hipLaunchKernelGGL((k_weights_rows<D, O, PT>), g, b, 0, s, pos, ngb, nq, k, h, indices, box, weighted, G, status)
+    if (order == 1) {
+        if (dim == 1) MHIP_W(1, 1);
+        if (dim == 2) MHIP_W(2, 1);
+        if (dim == 3) MHIP_W(3, 1);
+    } else {
+        if (dim == 1) MHIP_W(1, 2);
+        if (dim == 2) MHIP_W(2, 2);
+        if (dim == 3) MHIP_W(3, 2);
+    }
+#undef MHIP_W
+}
+
+int weights_rows(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim, const int32_t *ngb,
+                 int64_t nq, int k, const double *h, const int64_t *indices, double box, int weighted,
+                 int order, double *G, int32_t *status) {
+    (void)n;
+    if (dim < 1 || dim > 3) return fail(ctx, -5, "dim must be 1, 2 or 3");
+    if (order != 1 && order != 2) return fail(ctx, -1, "order must be 1 or 2");
+    if (nq < 1 || k < 1) return 0;
+    if (pos_f64)
+        dispatch_weights<double>((const double *)pos, dim, order, ngb, nq, k, h, indices, box, weighted,
+                                 G, status, ctx->stream);
+    else
+        dispatch_weights<float>((const float *)pos, dim, order, ngb, nq, k, h, indices, box, weighted, G,
+                                status, ctx->stream);
+    return check_hip(ctx, hipGetLastError(), "weights launch");
+}
+
+// ---------------------------------------------------------------------------
+// HsmlIter on a given distance array (kernel_density.py:14-57): float64,
+// bracket [d_1, c_d d_{k-1}], stop when |n - k| <= error_norm.  Safeguarded
+// Newton replaces the plain bisection; the root is the same.
+// ---------------------------------------------------------------------------
+template <typename DT>
+__global__ __launch_bounds__(128) void k_hsml_dists(const DT *__restrict__ d, int64_t n, int k, int dim,
+                                                   double error_norm, double *__restrict__ hout,
+                                                   int32_t *__restrict__ status) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DT *row = d + i * k;
+    const double norm = hsml_norm(dim);
+    double lo = (double)row[1], hi = (double)row[k - 1] * hsml_bound_coeff(dim);
+    double h = 0.5 * (lo + hi);
+    int32_t st = 0;
+    int it = 0;
+    for (; it < 200; it++) {
+        const double ih = 1.0 / h;
+        double s = 0.0, ds = 0.0;
+        for (int j = 0; j < k; j++) {
+            const double q = (double)row[j] * ih;
+            double w, dw;
+            kernel_w_dw<double>(q, w, dw);
+            s += w;
+            if (q <= 1.0) ds += dw * q;
+        }
+        const double f = norm * s - (double)k;
+        if (fabs(f) <= error_norm) break;
+        if (f > 0)
+            hi = h;
+        else
+            lo = h;
+        double hn = h - f / (-norm * ds * ih);
+        if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
+        if (hn == h) break;
+        h = hn;
+    }
+    if (it >= 200) st |= ST_HSML_NOCONV;
+    hout[i] = h;
+    if (status) status[i] = st;
+}
+
+int hsml_dists(mhip_ctx *ctx, const void *d, int d_f64, int64_t n, int k, int dim, double error_norm,
+               double *h, int32_t *status) {
+    if (k < 2) return fail(ctx, -1, "need at least 2 neighbour distances per row");
+    if (dim < 1 || dim > 3) return fail(ctx, -5, "dim must be 1, 2 or 3");
+    if (n < 1) return 0;
+    const dim3 g((unsigned)((n + 127) / 128)), b(128);
+    if (d_f64)
+        hipLaunchKernelGGL(k_hsml_dists<double>, g, b, 0, ctx->stream, (const double *)d, n, k, dim,
+                           error_norm, h, status);
+    else
+        hipLaunchKernelGGL(k_hsml_dists<float>, g, b, 0, ctx->stream, (const float *)d, n, k, dim,
+                           error_norm, h, status);
+    return check_hip(ctx, hipGetLastError(), "hsml launch");
+}
+
+// Kernel(q) ufunc (kernel_density.py:82-96)
+template <typename T>
+__global__ void k_kernel_eval(const T *__restrict__ q, int64_t n, T *__restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = kernel_w<T>(q[i]);
+}
+
+int kernel_eval(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out) {
+    if (n < 1) return 0;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    if (f64)
+        hipLaunchKernelGGL(k_kernel_eval<double>, g, b, 0, ctx->stream, (const double *)q, n, (double *)out);
+    else
+        hipLaunchKernelGGL(k_kernel_eval<float>, g, b, 0, ctx->stream, (const float *)q, n, (float *)out);
+    return check_hip(ctx, hipGetLastError(), "kernel launch");
+}
+
+}  // namespace mhip

@@ -1,0 +1,462 @@
+// Spatial index: the GPU replacement of scipy's cKDTree build
+// (src/meshoid/Meshoid.py:142-145).  Particles are sorted by a 21-bit-per-axis
+// Morton key with a project-written LSD radix sort and reordered into float32
+// structure-of-arrays; an implicit tree groups LEAF consecutive particles into
+// leaves and FANOUT consecutive nodes into parents, each with its bounding box.
+#include <algorithm>
+
+#include "mhip_internal.hpp"
+
+namespace mhip {
+
+// ---------------------------------------------------------------------------
+// bounding box
+// ---------------------------------------------------------------------------
+template <typename PT>
+__global__ __launch_bounds__(256) void k_bbox_partial(const PT *__restrict__ pos, int64_t n, int dim,
+                                                     float *__restrict__ part) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float nanflag = 0.f;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        for (int c = 0; c < dim; c++) {
+            float v = (float)pos[i * dim + c];
+            if (v != v) nanflag = 1.f;
+            lo[c] = fminf(lo[c], v);
+            hi[c] = fmaxf(hi[c], v);
+        }
+    }
+    __shared__ float s[7][256];
+    for (int c = 0; c < 3; c++) {
+        s[c][threadIdx.x] = lo[c];
+        s[3 + c][threadIdx.x] = hi[c];
+    }
+    s[6][threadIdx.x] = nanflag;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int c = 0; c < 3; c++) {
+                s[c][threadIdx.x] = fminf(s[c][threadIdx.x], s[c][threadIdx.x + w]);
+                s[3 + c][threadIdx.x] = fmaxf(s[3 + c][threadIdx.x], s[3 + c][threadIdx.x + w]);
+            }
+            s[6][threadIdx.x] = fmaxf(s[6][threadIdx.x], s[6][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) part[blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// single block: reduce partials into out[0..6] (lo xyz, hi xyz, nan flag) and
+// the quantisation frame frame[0..3] (origin xyz, scale)
+__global__ __launch_bounds__(256) void k_bbox_final(const float *__restrict__ part, int nb, int dim,
+                                                   float box, float *__restrict__ out) {
+    __shared__ float s[7][256];
+    float v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.f};
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        for (int c = 0; c < 3; c++) {
+            v[c] = fminf(v[c], part[b * 8 + c]);
+            v[3 + c] = fmaxf(v[3 + c], part[b * 8 + 3 + c]);
+        }
+        v[6] = fmaxf(v[6], part[b * 8 + 6]);
+    }
+    for (int c = 0; c < 7; c++) s[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int c = 0; c < 3; c++) {
+                s[c][threadIdx.x] = fminf(s[c][threadIdx.x], s[c][threadIdx.x + w]);
+                s[3 + c][threadIdx.x] = fmaxf(s[3 + c][threadIdx.x], s[3 + c][threadIdx.x + w]);
+            }
+            s[6][threadIdx.x] = fmaxf(s[6][threadIdx.x], s[6][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 7; c++) out[c] = s[c][0];
+        for (int c = dim; c < 3; c++) {
+            out[c] = 0.f;
+            out[3 + c] = 0.f;
+        }
+        // frame: cubic cells over the longest side (or the periodic box)
+        float ext = 0.f;
+        for (int c = 0; c < dim; c++) ext = fmaxf(ext, out[3 + c] - out[c]);
+        float org[3] = {out[0], out[1], out[2]};
+        if (box > 0.f) {
+            ext = box;
+            org[0] = org[1] = org[2] = 0.f;
+        }
+        for (int c = 0; c < 3; c++) out[8 + c] = org[c];
+        out[11] = ext > 0.f ? 2097151.0f / ext : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Morton keys (21 bits per axis, x in bit 0)
+// ---------------------------------------------------------------------------
+MHIP_DEV uint64_t spread21(uint64_t x) {
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+template <typename PT>
+__global__ __launch_bounds__(256) void k_morton(const PT *__restrict__ pos, int64_t n, int dim,
+                                               const float *__restrict__ bb,
+                                               uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float scale = bb[11];
+    uint64_t key = 0;
+    for (int c = 0; c < dim; c++) {
+        float t = ((float)pos[i * dim + c] - bb[8 + c]) * scale;
+        t = fminf(fmaxf(t, 0.f), 2097151.0f);
+        key |= spread21((uint64_t)t) << c;
+    }
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits.  Per pass: block histograms (digit-major),
+// per-digit scan across blocks, then a stable scatter in which each wavefront
+// ranks its 64 keys by digit with ballot-based matching.
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+size_t radix_counts_words(int64_t n) {
+    int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    return (size_t)(256 * std::max<int64_t>(nb, 1) + 256);
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys, int64_t n,
+                                                        int shift, uint32_t *__restrict__ counts,
+                                                        int nblocks) {
+    __shared__ uint32_t h[4][256];
+    const int tid = threadIdx.x, w = tid >> 6;
+    for (int i = tid; i < 1024; i += RS_THREADS) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t idx = base + r * RS_THREADS + tid;
+        if (idx < n) atomicAdd(&h[w][(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(int64_t)tid * nblocks + blockIdx.x] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
+}
+
+// one block per digit: exclusive scan of counts[d][0..nblocks), total -> totals[d]
+__global__ __launch_bounds__(256) void k_rs_scan(uint32_t *__restrict__ counts, int nblocks,
+                                                 uint32_t *__restrict__ totals) {
+    __shared__ uint32_t s[256];
+    uint32_t *row = counts + (int64_t)blockIdx.x * nblocks;
+    const int tid = threadIdx.x;
+    const int per = (nblocks + 255) / 256;
+    const int b0 = tid * per, b1 = min(b0 + per, nblocks);
+    uint32_t sum = 0;
+    for (int b = b0; b < b1; b++) sum += row[b];
+    s[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = tid >= off ? s[tid - off] : 0u;
+        __syncthreads();
+        s[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[tid] - sum;
+    for (int b = b0; b < b1; b++) {
+        uint32_t c = row[b];
+        row[b] = run;
+        run += c;
+    }
+    if (tid == 255) totals[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ kin,
+                                                           const uint32_t *__restrict__ vin, int64_t n,
+                                                           int shift, const uint32_t *__restrict__ counts,
+                                                           const uint32_t *__restrict__ totals,
+                                                           int nblocks, uint64_t *__restrict__ kout,
+                                                           uint32_t *__restrict__ vout) {
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_w[4][256];
+    __shared__ uint32_t s_tmp[256];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    // digit base = exclusive scan of totals + this block's scanned count
+    s_tmp[tid] = totals[tid];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = tid >= off ? s_tmp[tid - off] : 0u;
+        __syncthreads();
+        s_tmp[tid] += v;
+        __syncthreads();
+    }
+    s_off[tid] = s_tmp[tid] - totals[tid] + counts[(int64_t)tid * nblocks + blockIdx.x];
+    for (int i = tid; i < 1024; i += RS_THREADS) (&s_w[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t idx = base + r * RS_THREADS + tid;
+        bool valid = idx < n;
+        uint64_t key = valid ? kin[idx] : 0ull;
+        uint32_t val = valid ? vin[idx] : 0u;
+        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank = __popcll(peers & lt_mask);
+        if (valid && rank == 0) s_w[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = s_off[d] + rank;
+            for (int ww = 0; ww < w; ww++) pos += s_w[ww][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        s_off[tid] += s_w[0][tid] + s_w[1][tid] + s_w[2][tid] + s_w[3][tid];
+        s_w[0][tid] = s_w[1][tid] = s_w[2][tid] = s_w[3][tid] = 0;
+        __syncthreads();
+    }
+}
+
+int radix_sort_pairs(mhip_ctx *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1,
+                     uint32_t *counts, int64_t n, int bits, uint64_t **kout, uint32_t **vout) {
+    uint64_t *ka = k0, *kb = k1;
+    uint32_t *va = v0, *vb = v1;
+    if (n > 1) {
+        const int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
+        uint32_t *totals = counts + (size_t)256 * nblocks;
+        for (int shift = 0; shift < bits; shift += 8) {
+            hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, ka, n,
+                               shift, counts, nblocks);
+            hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, ctx->stream, counts, nblocks,
+                               totals);
+            hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, ka,
+                               va, n, shift, counts, totals, nblocks, kb, vb);
+            std::swap(ka, kb);
+            std::swap(va, vb);
+        }
+    }
+    *kout = ka;
+    *vout = va;
+    return check_hip(ctx, hipGetLastError(), "radix sort");
+}
+
+// ---------------------------------------------------------------------------
+// reorder into sorted space
+// ---------------------------------------------------------------------------
+template <typename PT>
+__global__ __launch_bounds__(256) void k_gather_sorted(const PT *__restrict__ pos, int64_t n, int64_t npad,
+                                                      int dim, const uint32_t *__restrict__ order,
+                                                      float *__restrict__ sx, float *__restrict__ sy,
+                                                      float *__restrict__ sz, float *__restrict__ lx,
+                                                      float *__restrict__ ly, float *__restrict__ lz,
+                                                      uint32_t *__restrict__ perm,
+                                                      uint32_t *__restrict__ iperm) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= npad) return;
+    const float nan = __builtin_nanf("");
+    float h[3] = {nan, nan, nan}, l[3] = {0.f, 0.f, 0.f};
+    if (p < n) {
+        uint32_t o = order[p];
+        perm[p] = o;
+        iperm[o] = (uint32_t)p;
+        for (int c = 0; c < 3; c++) {
+            if (c < dim) {
+                PT v = pos[(int64_t)o * dim + c];
+                h[c] = (float)v;
+                l[c] = (float)(v - (PT)h[c]);
+            } else {
+                h[c] = 0.f;
+            }
+        }
+    } else {
+        perm[p] = 0xffffffffu;
+    }
+    sx[p] = h[0];
+    sy[p] = h[1];
+    sz[p] = h[2];
+    if (lx) {
+        lx[p] = l[0];
+        ly[p] = l[1];
+        lz[p] = l[2];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_leaf_boxes(const float *__restrict__ sx, const float *__restrict__ sy,
+                                                   const float *__restrict__ sz, int64_t n, int64_t nleaf,
+                                                   float *__restrict__ nodes) {
+    int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (L >= nleaf) return;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int64_t b = L * LEAF, e = min<int64_t>(b + LEAF, n);
+    for (int64_t p = b; p < e; p++) {
+        float v[3] = {sx[p], sy[p], sz[p]};
+        for (int c = 0; c < 3; c++) {
+            lo[c] = fminf(lo[c], v[c]);
+            hi[c] = fmaxf(hi[c], v[c]);
+        }
+    }
+    float *o = nodes + L * 8;
+    o[0] = lo[0];
+    o[1] = lo[1];
+    o[2] = lo[2];
+    o[3] = hi[0];
+    o[4] = hi[1];
+    o[5] = hi[2];
+    o[6] = 0.f;
+    o[7] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_node_boxes(const float *__restrict__ child, int64_t nchild,
+                                                   float *__restrict__ parent, int64_t nparent) {
+    int64_t P = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (P >= nparent) return;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int64_t b = P * FANOUT, e = min<int64_t>(b + FANOUT, nchild);
+    for (int64_t c = b; c < e; c++) {
+        const float *s = child + c * 8;
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(lo[a], s[a]);
+            hi[a] = fmaxf(hi[a], s[3 + a]);
+        }
+    }
+    float *o = parent + P * 8;
+    o[0] = lo[0];
+    o[1] = lo[1];
+    o[2] = lo[2];
+    o[3] = hi[0];
+    o[4] = hi[1];
+    o[5] = hi[2];
+    o[6] = 0.f;
+    o[7] = 0.f;
+}
+
+static int grow(mhip_ctx *ctx, void **p, size_t bytes) {
+    if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+    if (bytes == 0) return 0;
+    return check_hip(ctx, hipMalloc(p, bytes), "hipMalloc (index)");
+}
+
+int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim, double box,
+                mhip_index *ix) {
+    if (n < 1) return fail(ctx, -1, "index needs at least one particle");
+    if (n >= (int64_t)0xffffffff) return fail(ctx, -4, "more than 2^32-2 particles");
+    if (dim < 1 || dim > 3) return fail(ctx, -5, "dim must be 1, 2 or 3");
+    const int64_t npad = (n + LEAF - 1) / LEAF * LEAF;
+    const int64_t nleaf = npad / LEAF;
+    int rc = 0;
+    // level sizes
+    int nlev = 0;
+    int64_t lev_n[MAX_LEVELS], lev_off[MAX_LEVELS], tot = 0;
+    for (int64_t m = nleaf;; m = (m + FANOUT - 1) / FANOUT) {
+        if (nlev == MAX_LEVELS) return fail(ctx, -4, "tree too deep");
+        lev_n[nlev] = m;
+        lev_off[nlev] = tot;
+        tot += m;
+        nlev++;
+        if (m == 1) break;
+    }
+    if (ix->cap < npad || (pos_f64 && !ix->lx)) {
+        void **arrs[6] = {(void **)&ix->sx, (void **)&ix->sy, (void **)&ix->sz,
+                          (void **)&ix->lx, (void **)&ix->ly, (void **)&ix->lz};
+        for (int a = 0; a < 6; a++)
+            if ((rc = grow(ctx, arrs[a], (a < 3 || pos_f64) ? npad * sizeof(float) : 0))) return rc;
+        if ((rc = grow(ctx, (void **)&ix->perm, npad * sizeof(uint32_t)))) return rc;
+        if ((rc = grow(ctx, (void **)&ix->iperm, npad * sizeof(uint32_t)))) return rc;
+        ix->cap = npad;
+    }
+    if (ix->node_cap < tot) {
+        if ((rc = grow(ctx, (void **)&ix->nodes, tot * 8 * sizeof(float)))) return rc;
+        ix->node_cap = tot;
+    }
+    ix->n = n;
+    ix->npad = npad;
+    ix->dim = dim;
+    ix->box = box;
+    ix->pos_f64 = pos_f64;
+    ix->pos_in = pos;
+    ix->nlev = nlev;
+    for (int l = 0; l < nlev; l++) {
+        ix->lev_n[l] = lev_n[l];
+        ix->lev_off[l] = lev_off[l];
+    }
+    // scratch layout: keys x2, vals x2, counts, bbox partials
+    const size_t cw = radix_counts_words(n);
+    const int nbb = 1024;
+    size_t off_k0 = 0, off_k1 = off_k0 + n * 8, off_v0 = off_k1 + n * 8, off_v1 = off_v0 + n * 4,
+           off_c = off_v1 + n * 4, off_bb = off_c + cw * 4, total = off_bb + (nbb + 2) * 8 * 4;
+    char *s = (char *)scratch(ctx, total, &rc);
+    if (!s) return rc;
+    uint64_t *k0 = (uint64_t *)(s + off_k0), *k1 = (uint64_t *)(s + off_k1);
+    uint32_t *v0 = (uint32_t *)(s + off_v0), *v1 = (uint32_t *)(s + off_v1);
+    uint32_t *counts = (uint32_t *)(s + off_c);
+    float *part = (float *)(s + off_bb), *bb = part + nbb * 8;
+    const int nb = (int)std::min<int64_t>(nbb, (n + 255) / 256);
+    const dim3 g1((unsigned)((n + 255) / 256)), b256(256);
+    if (pos_f64) {
+        hipLaunchKernelGGL(k_bbox_partial<double>, dim3(nb), b256, 0, ctx->stream, (const double *)pos,
+                           n, dim, part);
+    } else {
+        hipLaunchKernelGGL(k_bbox_partial<float>, dim3(nb), b256, 0, ctx->stream, (const float *)pos,
+                           n, dim, part);
+    }
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), b256, 0, ctx->stream, part, nb, dim, (float)box, bb);
+    if (pos_f64)
+        hipLaunchKernelGGL(k_morton<double>, g1, b256, 0, ctx->stream, (const double *)pos, n, dim, bb,
+                           k0, v0);
+    else
+        hipLaunchKernelGGL(k_morton<float>, g1, b256, 0, ctx->stream, (const float *)pos, n, dim, bb,
+                           k0, v0);
+    uint64_t *ks;
+    uint32_t *vs;
+    if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, 21 * dim, &ks, &vs))) return rc;
+    const dim3 gp((unsigned)((npad + 255) / 256));
+    if (pos_f64)
+        hipLaunchKernelGGL(k_gather_sorted<double>, gp, b256, 0, ctx->stream, (const double *)pos, n,
+                           npad, dim, vs, ix->sx, ix->sy, ix->sz, ix->lx, ix->ly, ix->lz, ix->perm,
+                           ix->iperm);
+    else
+        hipLaunchKernelGGL(k_gather_sorted<float>, gp, b256, 0, ctx->stream, (const float *)pos, n,
+                           npad, dim, vs, ix->sx, ix->sy, ix->sz, (float *)nullptr, (float *)nullptr,
+                           (float *)nullptr, ix->perm, ix->iperm);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3((unsigned)((nleaf + 255) / 256)), b256, 0, ctx->stream,
+                       ix->sx, ix->sy, ix->sz, n, nleaf, ix->nodes);
+    for (int l = 1; l < nlev; l++)
+        hipLaunchKernelGGL(k_node_boxes, dim3((unsigned)((lev_n[l] + 255) / 256)), b256, 0,
+                           ctx->stream, ix->nodes + lev_off[l - 1] * 8, lev_n[l - 1],
+                           ix->nodes + lev_off[l] * 8, lev_n[l]);
+    if ((rc = check_hip(ctx, hipGetLastError(), "index build launch"))) return rc;
+    // bbox + nan flag back to the host: periodic range and NaN checks
+    float hb[8];
+    if ((rc = check_hip(ctx, hipMemcpyAsync(hb, bb, 7 * sizeof(float), hipMemcpyDeviceToHost,
+                                            ctx->stream), "bbox readback")))
+        return rc;
+    if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "index build"))) return rc;
+    for (int c = 0; c < 6; c++) ix->bbox[c] = hb[c];
+    if (hb[6] != 0.f) return fail(ctx, -1, "positions contain NaN");
+    if (box > 0) {
+        for (int c = 0; c < dim; c++)
+            if (hb[c] < 0.f || hb[3 + c] >= (float)box)
+                return fail(ctx, -1,
+                            "Some input data are greater than the size of the periodic box.");
+    }
+    return 0;
+}
+
+}  // namespace mhip

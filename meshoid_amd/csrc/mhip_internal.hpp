@@ -1,0 +1,43 @@
+// Internal (non-exported) launch helpers shared by the .hip translation units.
+#pragma once
+#include "mhip_common.hpp"
+
+namespace mhip {
+
+// error plumbing: every launcher returns 0 or a negative MHIP_ERR_* code and
+// records a message in ctx->last_error
+int fail(mhip_ctx *ctx, int code, const std::string &msg);
+int check_hip(mhip_ctx *ctx, hipError_t e, const char *what);
+void *scratch(mhip_ctx *ctx, size_t bytes, int *rc);
+
+// LSD radix sort of (u64 key, u32 value) pairs on bits [0, bits) of the key.
+// Buffers: k0/v0 hold the input; the sorted result is returned in *kout/*vout
+// (one of the two ping-pong buffers).  counts needs 256*ceil(n/4096)+256 u32.
+int radix_sort_pairs(mhip_ctx *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1,
+                     uint32_t *counts, int64_t n, int bits, uint64_t **kout, uint32_t **vout);
+size_t radix_counts_words(int64_t n);
+
+// spatial index build (bbox -> keys -> sort -> reorder -> boxes)
+int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim, double box,
+                mhip_index *ix);
+
+// query setup + pass1/pass2/resolve
+int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
+            const double *m_rows, const float *seed_h, float *h_out, float *rho_out,
+            float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl);
+
+int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out);
+
+int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows,
+              int K, int order, int weighted, float *out1, float *out2, int32_t *status);
+
+int weights_rows(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
+                 const int32_t *ngb, int64_t nq, int k, const double *h, const int64_t *indices,
+                 double box, int weighted, int order, double *G, int32_t *status);
+
+int hsml_dists(mhip_ctx *ctx, const void *d, int d_f64, int64_t n, int k, int dim,
+               double error_norm, double *h, int32_t *status);
+
+int kernel_eval(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out);
+
+}  // namespace mhip

@@ -1,0 +1,691 @@
+// Exact k-nearest-neighbour search, smoothing length and density on the GPU.
+//
+// Replaces, for the hot path, the reference's
+//   cKDTree.query(pos[mask], des_ngb)           src/meshoid/Meshoid.py:157
+//   HsmlIter(ngbdist, error_norm=1e-13, dim)     src/meshoid/Meshoid.py:165,
+//                                                src/meshoid/kernel_density.py:14-57
+//   density = des_ngb*m/(volnorm*h^dim)          src/meshoid/Meshoid.py:169-170
+//
+// Design (one wavefront = 64 consecutive queries in Morton order):
+//   pass 1  wave-uniform traversal of the implicit tree (children near-to-far
+//           through a per-wave LDS stack); every lane keeps the k+1 smallest
+//           float32 squared distances in registers, inserted with v_med3 (one
+//           instruction per slot).  The epilogue solves the smoothing length on
+//           those k distances (safeguarded Newton, compensated float32 sums) and
+//           flags queries whose k-th and (k+1)-th distances are closer than
+//           float32 rounding can order.
+//   pass 2  same traversal with the now exact radius tau_i = k-th distance:
+//           collects exactly the k members (ties at tau capped by the count of
+//           members strictly inside) into a column-major list.
+//   resolve for flagged queries only: the members in the rounding band around
+//           tau are re-ranked with float64 distances computed from the
+//           caller's positions, so neighbour sets equal the float64 kNN except
+//           for float64 ties.
+#include "mhip_internal.hpp"
+
+namespace mhip {
+
+constexpr int KNN_BLOCK = 256;
+constexpr int KNN_WAVES = KNN_BLOCK / WAVE;
+constexpr int STACK = FANOUT * MAX_LEVELS;
+constexpr float U32 = 5.9604645e-8f;         // 2^-24
+constexpr float AMBIG_REL = 12.0f * U32;     // fp32 vs fp64 ordering ambiguity of d^2
+constexpr float BAND_REL = 16.0f * U32;      // resolver band half-width
+constexpr float PRUNE_REL = 1.0f + 1e-5f;    // box-test safety factor
+constexpr int BAND_CAP = 32;
+
+struct TreeView {
+    const float *sx, *sy, *sz, *lx, *ly, *lz;
+    const float *nodes;
+    int nlev;
+    int64_t lev_n[MAX_LEVELS], lev_off[MAX_LEVELS];
+    float box, half;  // periodic box length and half of it (0: open)
+    float slack;      // absolute box-test slack: 0 for float32 inputs, covers
+                      // the float32 residuals of float64 inputs otherwise
+};
+
+struct QPos {
+    float x, y, z, lx, ly, lz;
+};
+
+template <bool P64>
+MHIP_DEV QPos load_q(const TreeView &t, uint32_t p) {
+    QPos q;
+    q.x = t.sx[p];
+    q.y = t.sy[p];
+    q.z = t.sz[p];
+    q.lx = P64 ? t.lx[p] : 0.f;
+    q.ly = P64 ? t.ly[p] : 0.f;
+    q.lz = P64 ? t.lz[p] : 0.f;
+    return q;
+}
+
+// Offset c - q along one axis.  Periodic: the nearest image is formed by
+// moving whichever coordinate lies in the far half by one box length; that
+// subtraction (x - box with box/2 <= x < box) is exact (Sterbenz), so the
+// wrapped offset carries a single rounding, like an unwrapped one.
+template <bool PER>
+MHIP_DEV float axis_off(float c, float q, float box, float half) {
+    const float d = c - q;
+    if (!PER) return d;
+    const float up = (c - box) - q;  // image of c one box lower (c in the far half)
+    const float dn = c - (q - box);  // image of c one box higher (q in the far half)
+    return d > half ? up : (d < -half ? dn : d);
+}
+
+// squared distance, float32, fixed operation order (shared by every pass so
+// that the same pair always yields the same bits)
+template <bool PER, bool P64>
+MHIP_DEV float dist2(const TreeView &t, const QPos &q, float cx, float cy, float cz, float clx,
+                     float cly, float clz) {
+    float dx = axis_off<PER>(cx, q.x, t.box, t.half);
+    float dy = axis_off<PER>(cy, q.y, t.box, t.half);
+    float dz = axis_off<PER>(cz, q.z, t.box, t.half);
+    if (P64) {
+        dx = dx + (clx - q.lx);
+        dy = dy + (cly - q.ly);
+        dz = dz + (clz - q.lz);
+    }
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+template <bool PER>
+MHIP_DEV float axis_gap(float q, float lo, float hi, float box, float slack) {
+    float g = fmaxf(lo - q, q - hi);
+    if (PER) {
+        // the box one period lower / higher
+        g = fminf(g, fmaxf((lo - box) - q, q - (hi - box)));
+        g = fminf(g, fmaxf(lo - (q - box), (q - box) - hi));
+    }
+    return fmaxf(g - slack, 0.f);
+}
+
+// lower bound of the squared distance from q to any particle in box nd
+template <bool PER>
+MHIP_DEV float box_d2(const TreeView &t, const QPos &q, const float *nd) {
+    float dx = axis_gap<PER>(q.x, nd[0], nd[3], t.box, t.slack);
+    float dy = axis_gap<PER>(q.y, nd[1], nd[4], t.box, t.slack);
+    float dz = axis_gap<PER>(q.z, nd[2], nd[5], t.box, t.slack);
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+MHIP_DEV uint32_t node_id(int lev, int64_t i) { return ((uint32_t)lev << 28) | (uint32_t)i; }
+
+// Wave-uniform depth-first traversal.  thresh() returns this lane's current
+// squared search radius; leaf(L) processes leaf L.  Children of an expanded
+// node are pushed far-to-near relative to the wave's centre query, so the
+// nearest child is popped first.
+template <bool PER, class Thresh, class Leaf>
+MHIP_DEV void traverse(const TreeView &t, const QPos &q, const QPos &centre, uint32_t *stk,
+                       int64_t skip_lo, int64_t skip_hi, Thresh thresh, Leaf leaf) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    int sp = 0;
+    if (lane == 0) stk[0] = node_id(t.nlev - 1, 0);
+    sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        sp--;
+        const uint32_t node = uniform(stk[sp]);
+        const int lev = (int)(node >> 28);
+        const int64_t ix = node & 0x0fffffffu;
+        const float *nd = t.nodes + (t.lev_off[lev] + ix) * 8;
+        const float bd = box_d2<PER>(t, q, nd);
+        if (!__any(bd <= thresh() * PRUNE_REL)) continue;
+        if (lev == 0) {
+            if (ix >= skip_lo && ix <= skip_hi) continue;
+            leaf(ix);
+        } else {
+            const int64_t c0 = ix * FANOUT;
+            const int nc = (int)min<int64_t>(FANOUT, t.lev_n[lev - 1] - c0);
+            // rank children by their box distance to the centre query
+            float cd = INFINITY;
+            if (lane < nc) cd = box_d2<PER>(t, centre, t.nodes + (t.lev_off[lev - 1] + c0 + lane) * 8);
+            int rank = 0;
+#pragma unroll
+            for (int o = 0; o < FANOUT; o++) {
+                float od = __shfl(cd, o);
+                rank += (o < nc) && (od < cd || (od == cd && o < lane));
+            }
+            if (lane < nc) stk[sp + nc - 1 - rank] = node_id(lev - 1, c0 + lane);
+            __builtin_amdgcn_wave_barrier();
+            sp += nc;
+        }
+    }
+}
+
+template <int KCAP>
+MHIP_DEV void insert_sorted(float (&A)[KCAP], float x) {
+#pragma unroll
+    for (int j = KCAP - 1; j > 0; --j) A[j] = med3(A[j - 1], x, A[j]);
+    A[0] = fminf(A[0], x);
+}
+
+struct KnnArgs {
+    TreeView t;
+    const uint32_t *qsorted;  // null: slot q is sorted particle q
+    const uint32_t *qrow;     // null: row = perm[q]
+    const uint32_t *perm;
+    int64_t nq, n;
+    int k, dim;
+    const double *m_rows;  // per output row, null = 1
+    const float *seed_h;   // per output row, null = none
+    float *h_out, *rho_out, *vol_out;
+    int32_t *status_out;
+    // per slot scratch for pass 2
+    float *tau;
+    int32_t *nlt;
+    float *h_slot;
+    int32_t *st_slot;
+    uint32_t *nl_idx;
+    float *nl_d2;
+};
+
+MHIP_DEV int64_t slot_of_lane(const KnnArgs &a, int64_t *first) {
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t wave0 = ((int64_t)b * KNN_WAVES + (threadIdx.x >> 6)) * WAVE;
+    *first = wave0;
+    return wave0 + (threadIdx.x & (WAVE - 1));
+}
+
+// ---------------------------------------------------------------------------
+// smoothing length on the k nearest distances (kernel_density.py:36-56 root):
+// norm * sum_j W(r_j/h) = k, bracket [r_1, c_d r_{k-1}].  Safeguarded Newton
+// with compensated float32 sums; the bracket is the fallback.
+// ---------------------------------------------------------------------------
+template <int KCAP>
+MHIP_DEV float hsml_solve(const float (&A)[KCAP], int k, int dim, float seed, int32_t &st) {
+    const int e = KCAP - 1 - k;  // first real slot
+    const float norm = (float)hsml_norm(dim);
+    float r[KCAP - 1];
+    float lo = 0.f, rmax = 0.f;
+#pragma unroll
+    for (int j = 0; j < KCAP - 1; j++) {
+        r[j] = A[j] >= 0.f ? __builtin_sqrtf(A[j]) : INFINITY;
+        if (j == e + 1) lo = r[j];
+        if (j == KCAP - 2) rmax = r[j];
+    }
+    float hi = (float)hsml_bound_coeff(dim) * rmax;
+    if (!(hi > 0.f)) return 0.f;  // all k at distance 0: the reference returns 0
+    float h = rmax;
+    if (seed > 0.f) {
+        if (seed > lo && seed < hi)
+            h = seed;
+        else
+            st |= ST_SEED_MISS;
+    }
+    const float kf = (float)k;
+    int it = 0;
+    for (; it < 40; it++) {
+        const float ih = 1.0f / h;
+        float s = 0.f, c = 0.f, ds = 0.f;
+#pragma unroll
+        for (int j = 0; j < KCAP - 1; j++) {
+            float q = r[j] * ih, w, dw;
+            kernel_w_dw(q, w, dw);
+            float y = w - c, tt = s + y;
+            c = (tt - s) - y;
+            s = tt;
+            ds += q <= 1.0f ? dw * q : 0.0f;
+        }
+        const float f = norm * s - kf;
+        if (f == 0.f) break;
+        if (f > 0.f)
+            hi = h;
+        else
+            lo = h;
+        const float dndh = -norm * ds * ih;
+        float hn = h - f / dndh;
+        const bool newton = hn > lo && hn < hi;
+        if (!newton) hn = 0.5f * (lo + hi);
+        // n(h) is C^2 (W is C^2 at q = 1/2 and q = 1), so Newton converges
+        // quadratically: after a step below 1e-4 h the error of hn is at the
+        // float32 noise floor.  A collapsed bracket also ends the solve.
+        if ((newton && fabsf(hn - h) <= 1e-4f * h) || hi - lo <= 2.5e-7f * h) {
+            h = hn;
+            break;
+        }
+        h = hn;
+    }
+    if (it >= 40) st |= ST_HSML_NOCONV;
+    return h;
+}
+
+template <int KCAP, bool PER, bool P64>
+__global__ __launch_bounds__(KNN_BLOCK) void k_knn_pass1(KnnArgs a) {
+    __shared__ uint32_t s_stack[KNN_WAVES][STACK];
+    int64_t wave0;
+    const int64_t slot = slot_of_lane(a, &wave0);
+    if (wave0 >= a.nq) return;
+    const int64_t qs_slot = min<int64_t>(slot, a.nq - 1);
+    const uint32_t qp = a.qsorted ? a.qsorted[qs_slot] : (uint32_t)qs_slot;
+    const TreeView &t = a.t;
+    const QPos q = load_q<P64>(t, qp);
+    QPos c;
+    c.x = __shfl(q.x, 31);
+    c.y = __shfl(q.y, 31);
+    c.z = __shfl(q.z, 31);
+    c.lx = c.ly = c.lz = 0.f;
+
+    float A[KCAP];
+    const int e = KCAP - 1 - a.k;
+#pragma unroll
+    for (int j = 0; j < KCAP; j++) A[j] = j < e ? -1.0f : INFINITY;
+
+    auto leaf = [&](int64_t L) {
+        const int64_t base = L * LEAF;
+        float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
+#pragma unroll
+        for (int i = 0; i < LEAF; i++) {
+            cx[i] = t.sx[base + i];
+            cy[i] = t.sy[base + i];
+            cz[i] = t.sz[base + i];
+            clx[i] = P64 ? t.lx[base + i] : 0.f;
+            cly[i] = P64 ? t.ly[base + i] : 0.f;
+            clz[i] = P64 ? t.lz[base + i] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < LEAF; i++) {
+            const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
+            const bool ins = d2 < A[KCAP - 1];
+            if (__any(ins)) insert_sorted<KCAP>(A, ins ? d2 : INFINITY);
+        }
+    };
+    // prime with the leaves holding this wave's own queries
+    const uint32_t p_first = uniform(__shfl(qp, 0));
+    const uint32_t p_last = uniform(__shfl(qp, WAVE - 1));
+    const int64_t l0 = p_first / LEAF;
+    const int64_t l1 = min<int64_t>(p_last / LEAF, l0 + 7);
+    for (int64_t L = l0; L <= l1; L++) leaf(L);
+    traverse<PER>(t, q, c, s_stack[threadIdx.x >> 6], l0, l1, [&]() { return A[KCAP - 1]; }, leaf);
+
+    if (slot >= a.nq) return;
+    int32_t st = 0;
+    const float tau = A[KCAP - 2], tau1 = A[KCAP - 1];
+    if (tau1 <= tau * (1.0f + AMBIG_REL)) st |= ST_AMBIGUOUS;
+    int nlt = 0;
+#pragma unroll
+    for (int j = 0; j < KCAP - 1; j++) nlt += (A[j] >= 0.f && A[j] < tau);
+    const uint32_t row = a.qrow ? a.qrow[slot] : a.perm[slot];
+    const float seed = a.seed_h ? a.seed_h[row] : 0.f;
+    const float h = hsml_solve<KCAP>(A, a.k, a.dim, seed, st);
+    a.tau[slot] = tau;
+    a.nlt[slot] = nlt;
+    a.h_slot[slot] = h;
+    a.st_slot[slot] = st;
+    const double vn = a.dim == 3 ? 4.18879020478639098 : (a.dim == 2 ? 3.14159265358979324 : 2.0);
+    const double m = a.m_rows ? a.m_rows[row] : 1.0;
+    const double hd = (double)h, hp = a.dim == 3 ? hd * hd * hd : (a.dim == 2 ? hd * hd : hd);
+    const double rho = (double)a.k * m / (vn * hp);
+    if (a.h_out) a.h_out[row] = h;
+    if (a.rho_out) a.rho_out[row] = (float)rho;
+    if (a.vol_out) a.vol_out[row] = (float)(m / rho);
+}
+
+template <bool PER, bool P64>
+__global__ __launch_bounds__(KNN_BLOCK) void k_knn_pass2(KnnArgs a) {
+    __shared__ uint32_t s_stack[KNN_WAVES][STACK];
+    int64_t wave0;
+    const int64_t slot = slot_of_lane(a, &wave0);
+    if (wave0 >= a.nq) return;
+    const bool live = slot < a.nq;
+    const int64_t qs_slot = min<int64_t>(slot, a.nq - 1);
+    const uint32_t qp = a.qsorted ? a.qsorted[qs_slot] : (uint32_t)qs_slot;
+    const TreeView &t = a.t;
+    const QPos q = load_q<P64>(t, qp);
+    QPos c;
+    c.x = __shfl(q.x, 31);
+    c.y = __shfl(q.y, 31);
+    c.z = __shfl(q.z, 31);
+    c.lx = c.ly = c.lz = 0.f;
+    const float tau = a.tau[qs_slot];
+    int eq_left = a.k - a.nlt[qs_slot];
+    int cnt = 0;
+    const int64_t nq = a.nq;
+    auto leaf = [&](int64_t L) {
+        const int64_t base = L * LEAF;
+#pragma unroll 4
+        for (int i = 0; i < LEAF; i++) {
+            const float d2 = dist2<PER, P64>(t, q, t.sx[base + i], t.sy[base + i], t.sz[base + i],
+                                             P64 ? t.lx[base + i] : 0.f, P64 ? t.ly[base + i] : 0.f,
+                                             P64 ? t.lz[base + i] : 0.f);
+            bool take = d2 < tau;
+            if (d2 == tau && eq_left > 0) {
+                take = true;
+                eq_left--;
+            }
+            if (take && live && cnt < a.k) {
+                a.nl_idx[(int64_t)cnt * nq + slot] = (uint32_t)(base + i);
+                if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = d2;
+                cnt++;
+            }
+        }
+    };
+    traverse<PER>(t, q, c, s_stack[threadIdx.x >> 6], 1, 0, [&]() { return tau; }, leaf);
+    if (live && cnt != a.k) {
+        // never leave undefined indices behind: pad with the query itself
+        a.st_slot[slot] |= ST_COUNT;
+        for (int j = cnt; j < a.k; j++) {
+            a.nl_idx[(int64_t)j * nq + slot] = qp;
+            if (a.nl_d2) a.nl_d2[(int64_t)j * nq + slot] = INFINITY;
+        }
+    }
+}
+
+// float64 squared distance between original particles i and j
+template <typename PT, bool PER>
+MHIP_DEV double exact_d2(const PT *pos, int dim, uint32_t i, uint32_t j, double box) {
+    double s = 0.0;
+    for (int c = 0; c < dim; c++) {
+        double d = (double)pos[(int64_t)j * dim + c] - (double)pos[(int64_t)i * dim + c];
+        if (PER) d = wrap<double>(d, box);
+        s = __dadd_rn(s, __dmul_rn(d, d));
+    }
+    return s;
+}
+
+// thread per slot; only ambiguous slots do work
+template <bool PER, bool P64, typename PT>
+__global__ __launch_bounds__(64) void k_knn_resolve(KnnArgs a, const PT *__restrict__ pos_in) {
+    const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (slot >= a.nq) return;
+    if (!(a.st_slot[slot] & ST_AMBIGUOUS)) return;
+    const TreeView &t = a.t;
+    const uint32_t qp = a.qsorted ? a.qsorted[slot] : (uint32_t)slot;
+    const QPos q = load_q<P64>(t, qp);
+    const float tau = a.tau[slot];
+    const float band_lo = tau * (1.0f - BAND_REL), band_hi = tau * (1.0f + BAND_REL);
+    const uint32_t qo = a.perm[qp];
+    const int64_t nq = a.nq;
+    double bd[BAND_CAP];
+    uint32_t bi[BAND_CAP];
+    int nb = 0, cnt = 0;
+    int32_t st = a.st_slot[slot];
+    uint32_t stk[STACK];
+    int sp = 0;
+    stk[sp++] = node_id(t.nlev - 1, 0);
+    while (sp > 0) {
+        const uint32_t node = stk[--sp];
+        const int lev = (int)(node >> 28);
+        const int64_t ix = node & 0x0fffffffu;
+        const float *nd = t.nodes + (t.lev_off[lev] + ix) * 8;
+        if (box_d2<PER>(t, q, nd) > band_hi * PRUNE_REL) continue;
+        if (lev > 0) {
+            const int64_t c0 = ix * FANOUT;
+            const int nc = (int)min<int64_t>(FANOUT, t.lev_n[lev - 1] - c0);
+            for (int i = nc - 1; i >= 0; i--) stk[sp++] = node_id(lev - 1, c0 + i);
+            continue;
+        }
+        for (int i = 0; i < LEAF; i++) {
+            const int64_t p = ix * LEAF + i;
+            const float d2 = dist2<PER, P64>(t, q, t.sx[p], t.sy[p], t.sz[p], P64 ? t.lx[p] : 0.f,
+                                             P64 ? t.ly[p] : 0.f, P64 ? t.lz[p] : 0.f);
+            if (d2 < band_lo) {
+                if (cnt < a.k) {
+                    a.nl_idx[(int64_t)cnt * nq + slot] = (uint32_t)p;
+                    if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = d2;
+                }
+                cnt++;
+            } else if (d2 <= band_hi && p < a.n) {
+                const double e = exact_d2<PT, PER>(pos_in, a.dim, qo, a.perm[p], (double)t.box);
+                if (nb < BAND_CAP) {
+                    bd[nb] = e;
+                    bi[nb] = (uint32_t)p;
+                    nb++;
+                } else {
+                    st |= ST_BAND_OVERFLOW;
+                    int w = 0;  // replace the largest entry if this one is smaller
+                    for (int m = 1; m < BAND_CAP; m++)
+                        if (bd[m] > bd[w]) w = m;
+                    if (e < bd[w]) {
+                        bd[w] = e;
+                        bi[w] = (uint32_t)p;
+                    }
+                }
+            }
+        }
+    }
+    // order the band by (exact d^2, original index) and take what is missing
+    for (int i = 1; i < nb; i++) {
+        double v = bd[i];
+        uint32_t vi = bi[i];
+        int j = i - 1;
+        while (j >= 0 && (bd[j] > v || (bd[j] == v && a.perm[bi[j]] > a.perm[vi]))) {
+            bd[j + 1] = bd[j];
+            bi[j + 1] = bi[j];
+            j--;
+        }
+        bd[j + 1] = v;
+        bi[j + 1] = vi;
+    }
+    for (int i = 0; i < nb && cnt < a.k; i++, cnt++) {
+        a.nl_idx[(int64_t)cnt * nq + slot] = bi[i];
+        if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = (float)bd[i];
+    }
+    if (cnt != a.k) {
+        st |= ST_COUNT;
+        for (int j = cnt; j < a.k; j++) {
+            a.nl_idx[(int64_t)j * nq + slot] = qp;
+            if (a.nl_d2) a.nl_d2[(int64_t)j * nq + slot] = INFINITY;
+        }
+    } else {
+        st &= ~ST_COUNT;
+    }
+    a.st_slot[slot] = st;
+}
+
+__global__ void k_qsetup_mask(const int64_t *__restrict__ qidx, int64_t nq,
+                              const uint32_t *__restrict__ iperm, uint64_t *__restrict__ keys,
+                              uint32_t *__restrict__ vals) {
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= nq) return;
+    keys[r] = iperm[qidx[r]];
+    vals[r] = (uint32_t)r;
+}
+
+__global__ void k_qsetup_finish(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                                int64_t nq, uint32_t *__restrict__ qsorted, uint32_t *__restrict__ qrow) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nq) return;
+    qsorted[s] = (uint32_t)keys[s];
+    qrow[s] = vals[s];
+}
+
+__global__ void k_copy_status(const int32_t *__restrict__ st_slot, const uint32_t *__restrict__ qrow,
+                              const uint32_t *__restrict__ perm, int64_t nq, int32_t *__restrict__ out) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nq) return;
+    out[qrow ? qrow[s] : perm[s]] = st_slot[s];
+}
+
+static int nl_grow(mhip_ctx *ctx, void **p, size_t bytes) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    if (!bytes) return 0;
+    return check_hip(ctx, hipMalloc(p, bytes), "hipMalloc (neighbour list)");
+}
+
+template <int KCAP>
+static void launch_pass1(const KnnArgs &a, bool per, bool p64, dim3 g, hipStream_t s) {
+    if (per) {
+        if (p64)
+            hipLaunchKernelGGL((k_knn_pass1<KCAP, true, true>), g, dim3(KNN_BLOCK), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_knn_pass1<KCAP, true, false>), g, dim3(KNN_BLOCK), 0, s, a);
+    } else {
+        if (p64)
+            hipLaunchKernelGGL((k_knn_pass1<KCAP, false, true>), g, dim3(KNN_BLOCK), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_knn_pass1<KCAP, false, false>), g, dim3(KNN_BLOCK), 0, s, a);
+    }
+}
+
+int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
+            const double *m_rows, const float *seed_h, float *h_out, float *rho_out, float *vol_out,
+            int32_t *status_out, int flags, mhip_nlist *nl) {
+    if (k < 2) return fail(ctx, -1, "des_ngb must be at least 2");
+    if (k > 128) return fail(ctx, -4, "des_ngb above 128 is not supported by the GPU kNN");
+    if (k > ix->n) return fail(ctx, -1, "des_ngb exceeds the number of particles");
+    if (!qidx) nq = ix->n;
+    if (nq < 1) return fail(ctx, -1, "no query rows");
+    int rc = 0;
+    if (nl->cap < nq || nl->k < k) {
+        void **arr[] = {(void **)&nl->qsorted, (void **)&nl->qrow, (void **)&nl->tau,
+                        (void **)&nl->nlt,     (void **)&nl->h,    (void **)&nl->status};
+        for (auto p : arr)
+            if ((rc = nl_grow(ctx, p, nq * 4))) return rc;
+        if ((rc = nl_grow(ctx, (void **)&nl->idx, (size_t)nq * k * 4))) return rc;
+        if ((rc = nl_grow(ctx, (void **)&nl->d2, (size_t)nq * k * 4))) return rc;
+        nl->cap = nq;
+    }
+    nl->nq = nq;
+    nl->k = k;
+    nl->index = ix;
+    nl->has_d2 = (flags & 1) != 0;
+    // query slots in Morton order
+    const bool masked = qidx != nullptr;
+    nl->masked = masked;
+    if (masked) {
+        size_t cw = radix_counts_words(nq);
+        char *s = (char *)scratch(ctx, nq * 24 + cw * 4, &rc);
+        if (!s) return rc;
+        uint64_t *k0 = (uint64_t *)s, *k1 = k0 + nq;
+        uint32_t *v0 = (uint32_t *)(k1 + nq), *v1 = v0 + nq, *counts = v1 + nq;
+        hipLaunchKernelGGL(k_qsetup_mask, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                           ctx->stream, qidx, nq, ix->iperm, k0, v0);
+        int bits = 1;
+        while (bits < 32 && ((int64_t)1 << bits) < ix->n) bits++;
+        uint64_t *ks;
+        uint32_t *vs;
+        if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nq, bits, &ks, &vs))) return rc;
+        hipLaunchKernelGGL(k_qsetup_finish, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                           ctx->stream, ks, vs, nq, nl->qsorted, nl->qrow);
+    }
+    KnnArgs a;
+    a.t.sx = ix->sx;
+    a.t.sy = ix->sy;
+    a.t.sz = ix->sz;
+    a.t.lx = ix->lx;
+    a.t.ly = ix->ly;
+    a.t.lz = ix->lz;
+    a.t.nodes = ix->nodes;
+    a.t.nlev = ix->nlev;
+    for (int l = 0; l < MAX_LEVELS; l++) {
+        a.t.lev_n[l] = ix->lev_n[l];
+        a.t.lev_off[l] = ix->lev_off[l];
+    }
+    const bool per = ix->box > 0;
+    a.t.box = per ? (float)ix->box : 0.f;
+    a.t.half = per ? 0.5f * (float)ix->box : 0.f;
+    {
+        float m = per ? (float)ix->box : 0.f;
+        for (int c = 0; c < 6; c++) m = fmaxf(m, fabsf(ix->bbox[c]));
+        a.t.slack = ix->pos_f64 ? 2.4e-7f * m : 0.f;  // 4 ulp of the largest |coordinate|
+    }
+    a.qsorted = masked ? nl->qsorted : nullptr;
+    a.qrow = masked ? nl->qrow : nullptr;
+    a.perm = ix->perm;
+    a.nq = nq;
+    a.n = ix->n;
+    a.k = k;
+    a.dim = ix->dim;
+    a.m_rows = m_rows;
+    a.seed_h = seed_h;
+    a.h_out = h_out;
+    a.rho_out = rho_out;
+    a.vol_out = vol_out;
+    a.status_out = status_out;
+    a.tau = nl->tau;
+    a.nlt = nl->nlt;
+    a.h_slot = nl->h;
+    a.st_slot = nl->status;
+    a.nl_idx = nl->idx;
+    a.nl_d2 = nl->has_d2 ? nl->d2 : nullptr;
+    const bool p64 = ix->pos_f64 != 0;
+    const dim3 g((unsigned)((nq + KNN_BLOCK - 1) / KNN_BLOCK));
+    hipStream_t s = ctx->stream;
+    const int need = k + 1;
+    if (need <= 5)
+        launch_pass1<5>(a, per, p64, g, s);
+    else if (need <= 9)
+        launch_pass1<9>(a, per, p64, g, s);
+    else if (need <= 17)
+        launch_pass1<17>(a, per, p64, g, s);
+    else if (need <= 21)
+        launch_pass1<21>(a, per, p64, g, s);
+    else if (need <= 33)
+        launch_pass1<33>(a, per, p64, g, s);
+    else if (need <= 65)
+        launch_pass1<65>(a, per, p64, g, s);
+    else if (need <= 101)
+        launch_pass1<101>(a, per, p64, g, s);
+    else
+        launch_pass1<129>(a, per, p64, g, s);
+    if (per) {
+        if (p64)
+            hipLaunchKernelGGL((k_knn_pass2<true, true>), g, dim3(KNN_BLOCK), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_knn_pass2<true, false>), g, dim3(KNN_BLOCK), 0, s, a);
+    } else {
+        if (p64)
+            hipLaunchKernelGGL((k_knn_pass2<false, true>), g, dim3(KNN_BLOCK), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_knn_pass2<false, false>), g, dim3(KNN_BLOCK), 0, s, a);
+    }
+    const dim3 gr((unsigned)((nq + 63) / 64));
+    if (per) {
+        if (p64)
+            hipLaunchKernelGGL((k_knn_resolve<true, true, double>), gr, dim3(64), 0, s, a,
+                               (const double *)ix->pos_in);
+        else
+            hipLaunchKernelGGL((k_knn_resolve<true, false, float>), gr, dim3(64), 0, s, a,
+                               (const float *)ix->pos_in);
+    } else {
+        if (p64)
+            hipLaunchKernelGGL((k_knn_resolve<false, true, double>), gr, dim3(64), 0, s, a,
+                               (const double *)ix->pos_in);
+        else
+            hipLaunchKernelGGL((k_knn_resolve<false, false, float>), gr, dim3(64), 0, s, a,
+                               (const float *)ix->pos_in);
+    }
+    if (status_out)
+        hipLaunchKernelGGL(k_copy_status, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s,
+                           nl->status, a.qrow, ix->perm, nq, status_out);
+    return check_hip(ctx, hipGetLastError(), "knn launch");
+}
+
+// ---------------------------------------------------------------------------
+// export: rows in output-row order, ascending (distance, original index),
+// original particle indices -- the layout cKDTree.query returns
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_export(const uint32_t *__restrict__ idx, const float *__restrict__ d2,
+                                               const uint32_t *__restrict__ qrow,
+                                               const uint32_t *__restrict__ perm, int64_t nq, int k,
+                                               int32_t *__restrict__ ngb_out, float *__restrict__ dist_out) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nq) return;
+    const int64_t row = qrow ? qrow[s] : perm[s];
+    for (int j = 0; j < k; j++) {
+        const float dj = d2[(int64_t)j * nq + s];
+        const uint32_t oj = perm[idx[(int64_t)j * nq + s]];
+        int rank = 0;
+        for (int l = 0; l < k; l++) {
+            const float dl = d2[(int64_t)l * nq + s];
+            const uint32_t ol = perm[idx[(int64_t)l * nq + s]];
+            rank += (dl < dj) || (dl == dj && ol < oj);
+        }
+        if (ngb_out) ngb_out[row * k + rank] = (int32_t)oj;
+        if (dist_out) dist_out[row * k + rank] = __builtin_sqrtf(dj);
+    }
+}
+
+int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out) {
+    const mhip_index *ix = nl->index;
+    if (!ix || nl->nq < 1) return fail(ctx, -1, "neighbour list is empty");
+    if (!nl->has_d2) return fail(ctx, -1, "neighbour list was built without distances");
+    hipLaunchKernelGGL(k_export, dim3((unsigned)((nl->nq + 255) / 256)), dim3(256), 0, ctx->stream,
+                       nl->idx, nl->d2, nl->masked ? nl->qrow : nullptr, ix->perm, nl->nq, nl->k,
+                       ngb_out, dist_out);
+    return check_hip(ctx, hipGetLastError(), "export launch");
+}
+
+}  // namespace mhip

@@ -1,0 +1,212 @@
+"""Device-side engine: thin Python objects over the C ABI handles.
+
+PyTorch provides device memory and the stream; every computation happens in
+libmeshoid_hip.so.  All tensors passed here must live on the same CUDA (HIP)
+device and be contiguous.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import lib, raise_for
+
+_CTX: dict[int, ctypes.c_void_p] = {}
+
+
+def _dev_index(device) -> int:
+    d = torch.device(device) if device is not None else torch.device("cuda")
+    if d.type != "cuda":
+        raise ValueError(f"meshoid_amd runs on a HIP device, got {d}")
+    return d.index if d.index is not None else torch.cuda.current_device()
+
+
+def context(device=None) -> ctypes.c_void_p:
+    """Per-device context bound to torch's current stream of that device."""
+    idx = _dev_index(device)
+    ctx = _CTX.get(idx)
+    if ctx is None:
+        if not torch.cuda.is_available():
+            raise RuntimeError("no HIP device available: meshoid_amd needs an AMD GPU")
+        ctx = ctypes.c_void_p()
+        rc = lib.mhip_ctx_create(idx, None, ctypes.byref(ctx))
+        raise_for(rc, None, "mhip_ctx_create failed")
+        _CTX[idx] = ctx
+    lib.mhip_ctx_set_stream(ctx, ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream))
+    return ctx
+
+
+def ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _need(t, dtypes, name):
+    if t is None:
+        return
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype not in dtypes:
+        raise ValueError(f"{name} has dtype {t.dtype}, expected one of {dtypes}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+class SpatialIndex:
+    """Morton-sorted particle set + implicit tree (the cKDTree replacement)."""
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        raise_for(lib.mhip_index_create(ctypes.byref(self.h)), None, "index create")
+        self.pos = None
+        self.n = 0
+        self.dim = 0
+        self.boxsize = None
+
+    def build(self, pos: torch.Tensor, boxsize=None):
+        _need(pos, (torch.float32, torch.float64), "pos")
+        if pos.dim() == 1:
+            pos = pos[:, None]
+        n, dim = pos.shape
+        ctx = context(pos.device)
+        rc = lib.mhip_index_build(ctx, self.h, ptr(pos), int(pos.dtype == torch.float64), n, dim,
+                                  float(boxsize) if boxsize else -1.0)
+        raise_for(rc, ctx)
+        self.pos = pos  # the library reads it for float64 boundary resolution
+        self.n, self.dim, self.boxsize = n, dim, boxsize
+        return self
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.mhip_index_destroy(self.h)
+            self.h = None
+
+
+class NeighborList:
+    """Exact k-nearest-neighbour rows held on the device (opaque layout)."""
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        raise_for(lib.mhip_nlist_create(ctypes.byref(self.h)), None, "nlist create")
+        self.nq = 0
+        self.k = 0
+        self.index = None
+        self.keep_dist = False
+
+    def export(self, want_ngb=True, want_dist=True):
+        """(ngb int32 (nq,k), dist float32 (nq,k)) rows ascending, as cKDTree.query."""
+        dev = self.index.pos.device
+        ngb = torch.empty((self.nq, self.k), dtype=torch.int32, device=dev) if want_ngb else None
+        dist = torch.empty((self.nq, self.k), dtype=torch.float32, device=dev) if want_dist else None
+        ctx = context(dev)
+        raise_for(lib.mhip_nlist_export(ctx, self.h, ptr(ngb), ptr(dist)), ctx)
+        return ngb, dist
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.mhip_nlist_destroy(self.h)
+            self.h = None
+
+
+def knn_hsml(index: SpatialIndex, k: int, qidx=None, m_rows=None, seed_h=None, keep_dist=True,
+             nlist: NeighborList | None = None, out=None):
+    """kNN + smoothing length + density.  Returns dict(h, rho, vol, status, nlist)."""
+    dev = index.pos.device
+    _need(qidx, (torch.int64,), "qidx")
+    _need(m_rows, (torch.float64,), "m_rows")
+    _need(seed_h, (torch.float32,), "seed_h")
+    nq = index.n if qidx is None else qidx.numel()
+    if qidx is not None and nq and (int(qidx.min()) < 0 or int(qidx.max()) >= index.n):
+        raise IndexError("query index out of range")
+    if m_rows is not None and m_rows.numel() != nq:
+        raise ValueError("m_rows must have one entry per query row")
+    if seed_h is not None and seed_h.numel() != nq:
+        raise ValueError("seed_h must have one entry per query row")
+    if out is None:
+        out = {name: torch.empty(nq, dtype=torch.float32, device=dev) for name in ("h", "rho", "vol")}
+        out["status"] = torch.empty(nq, dtype=torch.int32, device=dev)
+    nl = nlist if nlist is not None else NeighborList()
+    ctx = context(dev)
+    rc = lib.mhip_knn_hsml(ctx, index.h, ptr(qidx), nq, int(k), ptr(m_rows), ptr(seed_h),
+                           ptr(out.get("h")), ptr(out.get("rho")), ptr(out.get("vol")),
+                           ptr(out.get("status")), _lib.KNN_KEEP_DIST if keep_dist else 0, nl.h)
+    raise_for(rc, ctx)
+    nl.nq, nl.k, nl.index, nl.keep_dist = nq, int(k), index, keep_dist
+    out["nlist"] = nl
+    return out
+
+
+def apply_D(nl: NeighborList, f: torch.Tensor, order=1, weighted=True, want_d1=True,
+            want_d2=False, out1=None, out2=None, status=None):
+    """Least-squares derivatives of f over the list's rows.
+
+    f: (nf_rows, K) contiguous float32/float64 device tensor.  Returns
+    (d1 (nq,K,dim) or None, d2 (nq,K,nd2) or None, status int32 (nq,))."""
+    _need(f, (torch.float32, torch.float64), "f")
+    if f.dim() == 1:
+        f = f[:, None]
+    nf, K = f.shape
+    dim = nl.index.dim
+    nd2 = {1: 1, 2: 3, 3: 6}[dim]
+    dev = f.device
+    if want_d1 and out1 is None:
+        out1 = torch.empty((nl.nq, K, dim), dtype=torch.float32, device=dev)
+    if want_d2 and out2 is None:
+        out2 = torch.empty((nl.nq, K, nd2), dtype=torch.float32, device=dev)
+    if status is None:
+        status = torch.empty(nl.nq, dtype=torch.int32, device=dev)
+    ctx = context(dev)
+    rc = lib.mhip_apply_D(ctx, nl.h, ptr(f), int(f.dtype == torch.float64), nf, K, int(order),
+                          int(bool(weighted)), ptr(out1 if want_d1 else None),
+                          ptr(out2 if want_d2 else None), ptr(status))
+    raise_for(rc, ctx)
+    return (out1 if want_d1 else None), (out2 if want_d2 else None), status
+
+
+def gradient_weights_dev(pos, ngb, h, indices, boxsize=None, weighted=True, order=1):
+    """G (nq, nd, k) float64 for row-major int32 ngb (device tensors)."""
+    _need(pos, (torch.float32, torch.float64), "pos")
+    _need(ngb, (torch.int32,), "ngb")
+    _need(h, (torch.float64,), "kernel_radius")
+    _need(indices, (torch.int64,), "indices")
+    if pos.dim() == 1:
+        pos = pos[:, None]
+    n, dim = pos.shape
+    nq, k = ngb.shape
+    if dim not in (1, 2, 3):
+        raise KeyError(dim)
+    nd = dim if order == 1 else {1: 2, 2: 5, 3: 9}[dim]
+    if indices.numel() != nq or h.numel() != nq:
+        raise ValueError("ngb, kernel_radius and indices must have the same number of rows")
+    if nq and (int(ngb.min()) < 0 or int(ngb.max()) >= n or int(indices.min()) < 0
+               or int(indices.max()) >= n):
+        raise IndexError("neighbour or particle index out of range")
+    G = torch.empty((nq, nd, k), dtype=torch.float64, device=pos.device)
+    status = torch.empty(nq, dtype=torch.int32, device=pos.device)
+    ctx = context(pos.device)
+    rc = lib.mhip_gradient_weights(ctx, ptr(pos), int(pos.dtype == torch.float64), n, dim, ptr(ngb),
+                                   nq, k, ptr(h), ptr(indices), float(boxsize) if boxsize else -1.0,
+                                   int(bool(weighted)), int(order), ptr(G), ptr(status))
+    raise_for(rc, ctx)
+    return G, status
+
+
+def hsml_iter_dev(d, dim=3, error_norm=1e-6):
+    _need(d, (torch.float32, torch.float64), "neighbor_dists")
+    n, k = d.shape
+    h = torch.empty(n, dtype=torch.float64, device=d.device)
+    status = torch.empty(n, dtype=torch.int32, device=d.device)
+    ctx = context(d.device)
+    rc = lib.mhip_hsml_iter(ctx, ptr(d), int(d.dtype == torch.float64), n, k, int(dim),
+                            float(error_norm), ptr(h), ptr(status))
+    raise_for(rc, ctx)
+    return h, status
+
+
+def kernel_dev(q):
+    _need(q, (torch.float32, torch.float64), "q")
+    out = torch.empty_like(q)
+    ctx = context(q.device)
+    raise_for(lib.mhip_kernel(ctx, ptr(q), int(q.dtype == torch.float64), q.numel(), ptr(out)), ctx)
+    return out

@@ -1,0 +1,316 @@
+"""``Meshoid`` -- drop-in for the reference class's neighbour-weighted operator
+path (src/meshoid/Meshoid.py:12-331), computed by libmeshoid_hip.so.
+
+Same constructor (``Meshoid(pos, m=None, kernel_radius=None, des_ngb=None,
+boxsize=None, verbose=False, particle_mask=None, n_jobs=-1)``; ``hsml=`` is an
+alias of ``kernel_radius``), same methods and attributes.  Attributes that the
+reference stores as NumPy arrays (``ngb``, ``ngbdist``, ``kernel_radius``,
+``density``, ``vol``, ``dweights``, ``d2weights``, ``dweights_3rdorder``) are
+produced from device results on first access; ``*_device`` accessors return
+the device tensors.  Known quirks of the reference are kept:
+
+* ``TreeUpdate`` multiplies ``self.m[self.particle_mask]`` although ``self.m``
+  is already masked (Meshoid.py:92 vs :169);
+* ``Curl``/``Div`` pass ``v[self.particle_mask]`` to ``D``, which indexes it
+  with global neighbour indices (Meshoid.py:310,330): IndexError when out of
+  range, as ``np.take`` raises;
+* ``Curl`` returns (dvy/dz - dvz/dy, dvx/dz - dvz/dx, dvx/dy - dvy/dx)
+  (Meshoid.py:311-315); ``Curl(v, correct_sign=True)`` gives the textbook curl;
+* the ``weighted`` flag of ``D``/``D2`` only matters for the first call of
+  each order, because the reference caches the weights (Meshoid.py:261-267);
+* ``Meshoid(pos, m, kernel_radius=h).D(f)`` re-runs the neighbour search and
+  replaces ``h`` by the adaptive smoothing length (Meshoid.py:255-256 -> 165).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import engine
+from ._lib import ST_SINGULAR
+from .functions import as_device, gradient_weights as _gradient_weights
+
+_UNSET = object()
+
+
+class Meshoid:
+    """Meshoid object that stores particle data and kernel-weighting
+    quantities, and implements derivative operators on the GPU."""
+
+    def __init__(self, pos, m=None, kernel_radius=None, des_ngb=None, boxsize=None, verbose=False,
+                 particle_mask=None, n_jobs=-1, *, hsml=None):
+        if kernel_radius is None and hsml is not None:
+            kernel_radius = hsml
+        self.tree = None
+        if len(pos.shape) == 1:
+            pos = pos[:, None]
+        self.verbose = verbose
+        self.N, self.dim = pos.shape
+        if particle_mask is None:
+            self.particle_mask = np.arange(self.N)
+        else:
+            pm = particle_mask.cpu().numpy() if isinstance(particle_mask, torch.Tensor) else particle_mask
+            if np.array(pm).dtype == np.dtype("bool"):
+                self.particle_mask = np.arange(len(pm))[pm]
+            else:
+                self.particle_mask = np.asarray(pm)
+        self.Nmask = len(self.particle_mask)
+        if des_ngb is None:
+            des_ngb = {1: 4, 2: 20, 3: 32}[self.dim]
+        self.des_ngb = des_ngb
+        self.n_jobs = n_jobs
+        self.volnorm = {1: 2.0, 2: np.pi, 3: 4 * np.pi / 3}[self.dim]
+        self.boxsize = boxsize
+        self.pos = pos
+        self._pos_d = as_device(pos)
+        self._center = self._L = None
+        if m is None:
+            m = np.repeat(1.0, self.N)
+        m = m.cpu().numpy() if isinstance(m, torch.Tensor) else np.asarray(m)
+        self.m = m[self.particle_mask]
+        self._identity_mask = self.Nmask == self.N and np.array_equal(
+            self.particle_mask, np.arange(self.N))
+        self._nl = None
+        self._dev = {}
+        self._np = {}
+        self._dw = {}
+        self._weighted = {}
+        self.sliceweights = None
+        if kernel_radius is None:
+            self.TreeUpdate()
+        else:
+            h = kernel_radius.cpu().numpy() if isinstance(kernel_radius, torch.Tensor) else kernel_radius
+            self._np["kernel_radius"] = h
+            self._np["vol"] = self.volnorm * h**self.dim / self.des_ngb
+            self._np["density"] = self.m / self._np["vol"]
+
+    # ------------------------------------------------------------------ state
+    def _cached(self, name):
+        if name in self._np:
+            return self._np[name]
+        if name in self._dev:
+            a = self._dev[name].cpu().numpy().astype(np.float64)
+            self._np[name] = a
+            return a
+        return None
+
+    kernel_radius = property(lambda s: s._cached("kernel_radius"),
+                             lambda s, v: s._set("kernel_radius", v))
+    density = property(lambda s: s._cached("density"), lambda s, v: s._set("density", v))
+    vol = property(lambda s: s._cached("vol"), lambda s, v: s._set("vol", v))
+
+    def _set(self, name, v):
+        self._dev.pop(name, None)
+        self._np[name] = v
+
+    @property
+    def ngb(self):
+        if "ngb" in self._np:
+            return self._np["ngb"]
+        if self._nl is None:
+            return None
+        self._export()
+        return self._np["ngb"]
+
+    @ngb.setter
+    def ngb(self, v):
+        self._np["ngb"] = v
+        if v is None:
+            self._nl = None
+
+    @property
+    def ngbdist(self):
+        if "ngbdist" in self._np:
+            return self._np["ngbdist"]
+        if self._nl is None:
+            raise AttributeError("'Meshoid' object has no attribute 'ngbdist'")
+        self._export()
+        return self._np["ngbdist"]
+
+    @ngbdist.setter
+    def ngbdist(self, v):
+        self._np["ngbdist"] = v
+
+    def _export(self):
+        ngb, dist = self._nl.export()
+        self._dev["ngb"], self._dev["ngbdist"] = ngb, dist
+        self._np["ngb"] = ngb.cpu().numpy().astype(np.int64)
+        self._np["ngbdist"] = dist.cpu().numpy().astype(np.float64)
+
+    @property
+    def center(self):
+        if self._center is None:
+            self._frame()
+        return self._center
+
+    @center.setter
+    def center(self, v):
+        self._center = v
+
+    @property
+    def L(self):
+        if self._L is None:
+            self._frame()
+        return self._L
+
+    @L.setter
+    def L(self, v):
+        self._L = v
+
+    def _frame(self):
+        """center/L of Meshoid.py:81-87 (computed on first use)."""
+        if self.boxsize is None:
+            p = np.asarray(self.pos.cpu() if isinstance(self.pos, torch.Tensor) else self.pos)
+            self._center = np.median(p, axis=0)
+            self._L = 2 * np.percentile(np.sum((p - self._center) ** 2, axis=1), 90) ** 0.5
+        else:
+            self._center = np.ones(3) * self.boxsize / 2
+            self._L = self.boxsize
+
+    def device_arrays(self):
+        """Device tensors of the latest TreeUpdate: h, rho, vol, status."""
+        return dict(self._dev)
+
+    # ------------------------------------------------------------ hot path
+    def reset_dweights(self):
+        self._dw.clear()
+        self._weighted.clear()
+
+    def BuildTree(self):
+        if self.verbose:
+            print("Building tree...")
+        self.tree = engine.SpatialIndex().build(self._pos_d, self.boxsize)
+
+    def TreeUpdate(self):
+        """Neighbour lists, smoothing lengths and densities (Meshoid.py:147-170)."""
+        if self.tree is None:
+            self.BuildTree()
+        if self.verbose:
+            print("Finding neighbors...")
+        dev = self._pos_d.device
+        m_rows = np.asarray(self.m[self.particle_mask], dtype=np.float64)  # Meshoid.py:169
+        m_d = torch.from_numpy(np.ascontiguousarray(m_rows)).to(dev)
+        qidx = None
+        if not self._identity_mask:
+            pm = np.asarray(self.particle_mask, dtype=np.int64)
+            if pm.size and (pm.min() < -self.N or pm.max() >= self.N):
+                raise IndexError("particle_mask index out of range")
+            qidx = torch.from_numpy(np.where(pm < 0, pm + self.N, pm)).to(dev)
+        out = engine.knn_hsml(self.tree, self.des_ngb, qidx=qidx, m_rows=m_d, keep_dist=True,
+                              nlist=self._nl)
+        if self.verbose:
+            print("Neighbours found!")
+            print("Iterating for smoothing lengths...")
+            print("Smoothing lengths found!")
+        self._nl = out["nlist"]
+        for key in ("ngb", "ngbdist", "kernel_radius", "density", "vol"):
+            self._np.pop(key, None)
+        self._dev = {"kernel_radius": out["h"], "density": out["rho"], "vol": out["vol"],
+                     "status": out["status"]}
+
+    def get_kernel_weights(self):
+        raise NotImplementedError("get_kernel_weights is outside the GPU hot path")
+
+    def Volume(self):
+        return self.vol
+
+    def NearestNeighbors(self):
+        if self.ngb is None:
+            self.TreeUpdate()
+        return self.ngb
+
+    def NeighborDistance(self):
+        if self.ngbdist is None:
+            self.TreeUpdate()
+        return self.ngbdist
+
+    def SmoothingLength(self):
+        if self.kernel_radius is None:
+            self.TreeUpdate()
+        return self.kernel_radius
+
+    def Density(self):
+        if self.density is None:
+            self.TreeUpdate()
+        return self.density
+
+    # ------------------------------------------------------ derivatives
+    def ComputeDWeights(self, order=1, weighted=True):
+        """Materialise the derivative weights (Meshoid.py:110-140)."""
+        if self.verbose:
+            print(f"Computing weights for derivatives of order {order}...")
+        self._weighted[order] = bool(weighted)
+        ngb = self.ngb
+        w = _gradient_weights(self._pos_d, as_device(ngb, (torch.int32,), torch.int32),
+                              as_device(self.kernel_radius, (torch.float64,)),
+                              as_device(self.particle_mask, (torch.int64,), torch.int64),
+                              boxsize=self.boxsize, weighted=weighted, order=order)
+        w = w.cpu().numpy()
+        if order == 1:
+            self._dw[1] = w
+        elif order == 2:
+            self._dw[2] = (w[:, self.dim:, :], w[:, : self.dim, :])
+
+    dweights = property(lambda s: s._dw.get(1), lambda s, v: s._dw.__setitem__(1, v))
+    d2weights = property(lambda s: s._dw[2][0] if 2 in s._dw else None)
+    dweights_3rdorder = property(lambda s: s._dw[2][1] if 2 in s._dw else None)
+
+    def _field(self, f):
+        """(nf, K) device tensor + tail shape; checks the np.take bounds."""
+        is_t = isinstance(f, torch.Tensor)
+        shape = tuple(f.shape)
+        if len(shape) == 0:
+            raise ValueError("f must have a leading particle axis")
+        fd = as_device(f)
+        fd = fd.reshape(shape[0], -1).contiguous()
+        nf = shape[0]
+        if nf < self.N:
+            # the reference indexes f with global neighbour indices (np.take)
+            if self.ngb is not None and self.ngb.size and int(self.ngb.max()) >= nf:
+                raise IndexError(f"index {int(self.ngb.max())} is out of bounds for axis 0 with size {nf}")
+            if self.particle_mask.size and int(np.max(self.particle_mask)) >= nf:
+                raise IndexError(f"index {int(np.max(self.particle_mask))} is out of bounds for axis 0 with size {nf}")
+        return fd, shape[1:], is_t
+
+    def _apply(self, f, order, weighted, second):
+        if self._nl is None:
+            self.TreeUpdate()
+        fd, tail, is_t = self._field(f)
+        w = self._weighted.setdefault(order, bool(weighted))
+        if self.verbose and order not in self._dw:
+            print(f"Computing weights for derivatives of order {order}...")
+        d1, d2, st = engine.apply_D(self._nl, fd, order=order, weighted=w, want_d1=not second,
+                                    want_d2=second)
+        if bool(((st & ST_SINGULAR) != 0).any()):
+            raise np.linalg.LinAlgError("Matrix is singular to machine precision.")
+        out = d2 if second else d1
+        out = out.reshape((self.Nmask,) + tuple(tail) + (out.shape[-1],))
+        if is_t:
+            return out
+        return out.cpu().numpy().astype(np.float64)
+
+    def D(self, f, order=1, weighted=True):
+        """Kernel-weighted least-squares gradient of f (Meshoid.py:237-268):
+        (Nmask, ..., dim)."""
+        return self._apply(f, order, weighted, second=False)
+
+    def D2(self, f, weighted=True):
+        """Least-squares second derivatives (Meshoid.py:270-295):
+        (Nmask, ..., nd2), 3D order [xx, yy, zz, xy, yz, zx]."""
+        return self._apply(f, 2, weighted, second=True)
+
+    def Curl(self, v, correct_sign=False):
+        """Curl of a vector field (Meshoid.py:297-315).  The reference's
+        component signs are kept unless correct_sign=True."""
+        dv = self.D(v[self.particle_mask])
+        c = [dv[:, 1, 2] - dv[:, 2, 1], dv[:, 0, 2] - dv[:, 2, 0], dv[:, 0, 1] - dv[:, 1, 0]]
+        if correct_sign:
+            c = [-c[0], c[1], -c[2]]
+        if isinstance(dv, torch.Tensor):
+            return torch.stack(c, dim=1)
+        return np.c_[c[0], c[1], c[2]]
+
+    def Div(self, v):
+        """Divergence of a vector field (Meshoid.py:317-331)."""
+        dv = self.D(v[self.particle_mask])
+        return dv[:, 0, 0] + dv[:, 1, 1] + dv[:, 2, 2]

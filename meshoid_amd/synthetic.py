@@ -1,0 +1,60 @@
+"""Seeded synthetic particle sets of the benchmark workloads (NumPy, host).
+
+* :func:`uniform`    -- positions uniform in the unit cube.
+* :func:`clustered`  -- mixture of isotropic Gaussian clumps: centres uniform in
+  a ``box``-unit cube, sigma log-uniform in [0.05, 5], clump weights from the
+  truncated power law dN/dM ~ M^-2 on [1, 1e4], particle counts multinomial;
+  optionally a uniform background fraction.  About six decades of density.
+* :func:`sin3`       -- f = sin(kx) sin(ky) sin(kz) and its exact gradient.
+* :func:`rotation_field` -- v = Omega x (r - c) + N(0, noise^2).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def uniform(n, dim=3, seed=0, dtype=np.float64):
+    return np.random.default_rng(seed).random((n, dim)).astype(dtype)
+
+
+def clump_params(nclump, seed, box=100.0, sig_lo=0.05, sig_hi=5.0, m_lo=1.0, m_hi=1e4):
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(0.0, box, (nclump, 3))
+    sigma = np.exp(rng.uniform(np.log(sig_lo), np.log(sig_hi), nclump))
+    u = rng.random(nclump)
+    # inverse CDF of p(M) ~ M^-2 on [m_lo, m_hi]
+    mass = 1.0 / (1.0 / m_lo - u * (1.0 / m_lo - 1.0 / m_hi))
+    return rng, centers, sigma, mass
+
+
+def clustered(n, seed=0, nclump=256, box=100.0, background=0.0, dtype=np.float32):
+    """(n, 3) positions; the first n_bg = round(background*n) rows are the
+    uniform background when background > 0."""
+    rng, centers, sigma, mass = clump_params(nclump, seed, box)
+    n_bg = int(round(background * n))
+    counts = rng.multinomial(n - n_bg, mass / mass.sum())
+    out = np.empty((n, 3), dtype=np.float64)
+    out[:n_bg] = rng.uniform(0.0, box, (n_bg, 3))
+    o = n_bg
+    for c, s, m in zip(centers, sigma, counts):
+        out[o:o + m] = c + s * rng.standard_normal((m, 3))
+        o += m
+    return out.astype(dtype)
+
+
+def sin3(pos, k=2 * np.pi / 10):
+    x = np.asarray(pos, dtype=np.float64)
+    s = np.sin(k * x)
+    c = np.cos(k * x)
+    f = s[:, 0] * s[:, 1] * s[:, 2]
+    g = np.stack([k * c[:, 0] * s[:, 1] * s[:, 2], k * s[:, 0] * c[:, 1] * s[:, 2],
+                  k * s[:, 0] * s[:, 1] * c[:, 2]], axis=1)
+    return f, g
+
+
+def rotation_field(pos, omega=(0.0, 0.0, 1.0), center=50.0, noise=0.1, seed=0):
+    x = np.asarray(pos, dtype=np.float64) - center
+    v = np.cross(np.asarray(omega, dtype=np.float64), x)
+    if noise:
+        v = v + noise * np.random.default_rng(seed).standard_normal(v.shape)
+    return v
