@@ -55,6 +55,15 @@ int mhip_ctx_synchronize(mhip_ctx *ctx);
 const char *mhip_last_error(const mhip_ctx *ctx);
 
 /*
+ * Kernel timing (no counterpart in the reference, whose notebooks use %time).
+ * While enabled, every launch region records HIP events on the context
+ * stream.  mhip_prof_read synchronises, writes one line per region,
+ * "name launches total_ms\n", into buf and clears the records.
+ */
+int mhip_prof_enable(mhip_ctx *ctx, int on);
+int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen);
+
+/*
  * Spatial index over n particles.  Replaces BuildTree / cKDTree(pos, boxsize)
  * (src/meshoid/Meshoid.py:142-145).  pos: (n, dim) row-major, float32
  * (pos_f64 = 0) or float64 (pos_f64 = 1); the buffer must stay alive while the

@@ -24,7 +24,7 @@ EXPORTED = [
     "mhip_ctx_synchronize", "mhip_last_error", "mhip_index_create", "mhip_index_build",
     "mhip_index_destroy", "mhip_index_info", "mhip_nlist_create", "mhip_nlist_destroy",
     "mhip_knn_hsml", "mhip_nlist_export", "mhip_apply_D", "mhip_gradient_weights",
-    "mhip_hsml_iter", "mhip_kernel",
+    "mhip_hsml_iter", "mhip_kernel", "mhip_prof_enable", "mhip_prof_read",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -60,6 +60,8 @@ _SIGS = {
                               _i),
     "mhip_hsml_iter": ([_p, _p, _i, _i64, _i, _i, _d, _p, _p], _i),
     "mhip_kernel": ([_p, _p, _i, _i64, _p], _i),
+    "mhip_prof_enable": ([_p, _i], _i),
+    "mhip_prof_read": ([_p, ctypes.c_char_p, _i64], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     _fn = getattr(lib, _name)

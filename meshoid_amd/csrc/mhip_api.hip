@@ -1,6 +1,7 @@
 // C ABI of libmeshoid_hip.so (declared in include/meshoid_hip.h): argument
 // checks, context/scratch management and error plumbing around the kernels.
 #include <stdio.h>
+#include <string.h>
 
 #include <new>
 
@@ -71,10 +72,56 @@ int mhip_ctx_create(int device, void *stream, mhip_ctx **out) {
 
 int mhip_ctx_destroy(mhip_ctx *ctx) {
     if (!ctx) return MHIP_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    if (ctx->scratch) hipFree(ctx->scratch);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    for (auto &r : ctx->recs) {
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     delete ctx;
+    return MHIP_OK;
+}
+
+int mhip_prof_enable(mhip_ctx *ctx, int on) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    ctx->prof = on ? 1 : 0;
+    return MHIP_OK;
+}
+
+int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen) {
+    if (!ctx || !buf || buflen < 1) return MHIP_ERR_VALUE;
+    int rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "profile synchronize");
+    if (rc) return rc;
+    // aggregate by region name, in first-seen order
+    std::vector<std::string> names;
+    std::vector<double> tot;
+    std::vector<long> cnt;
+    for (auto &r : ctx->recs) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.start, r.stop);
+        size_t i = 0;
+        while (i < names.size() && names[i] != r.name) i++;
+        if (i == names.size()) {
+            names.push_back(r.name);
+            tot.push_back(0.0);
+            cnt.push_back(0);
+        }
+        tot[i] += ms;
+        cnt[i]++;
+        ctx->event_pool.push_back(r.start);
+        ctx->event_pool.push_back(r.stop);
+    }
+    ctx->recs.clear();
+    std::string out;
+    char line[160];
+    for (size_t i = 0; i < names.size(); i++) {
+        snprintf(line, sizeof line, "%s %ld %.6f\n", names[i].c_str(), cnt[i], tot[i]);
+        out += line;
+    }
+    if ((int64_t)out.size() + 1 > buflen) return fail(ctx, MHIP_ERR_VALUE, "profile buffer too small");
+    memcpy(buf, out.c_str(), out.size() + 1);
     return MHIP_OK;
 }
 

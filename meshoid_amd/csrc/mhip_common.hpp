@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #define MHIP_DEV __device__ __forceinline__
 
@@ -101,14 +102,55 @@ MHIP_DEV uint32_t xcd_remap(uint32_t b, uint32_t nblocks) {
 // ---------------------------------------------------------------------------
 // host-side objects behind the opaque C handles
 // ---------------------------------------------------------------------------
+struct mhip_prof_rec {
+    const char *name;
+    hipEvent_t start, stop;
+};
+
 struct mhip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
-    void *hscratch = nullptr;  // small pinned host buffer for readbacks
+    // kernel timing: HIP events recorded on the context stream around each
+    // launch region while prof is on (mhip_prof_enable / mhip_prof_read)
+    int prof = 0;
+    std::vector<mhip_prof_rec> recs;
+    std::vector<hipEvent_t> event_pool;
 };
+
+namespace mhip {
+// records start/stop events around a launch region when profiling is on
+struct ProfRegion {
+    mhip_ctx *ctx;
+    int slot = -1;
+    ProfRegion(mhip_ctx *c, const char *name) : ctx(c) {
+        if (!ctx->prof) return;
+        mhip_prof_rec r;
+        r.name = name;
+        r.start = take(), r.stop = take();
+        (void)hipEventRecord(r.start, ctx->stream);
+        ctx->recs.push_back(r);
+        slot = (int)ctx->recs.size() - 1;
+    }
+    void end() {
+        if (slot >= 0) (void)hipEventRecord(ctx->recs[slot].stop, ctx->stream);
+        slot = -1;
+    }
+    ~ProfRegion() { end(); }
+    hipEvent_t take() {
+        hipEvent_t e;
+        if (!ctx->event_pool.empty()) {
+            e = ctx->event_pool.back();
+            ctx->event_pool.pop_back();
+        } else {
+            (void)hipEventCreate(&e);
+        }
+        return e;
+    }
+};
+}  // namespace mhip
 
 struct mhip_index {
     int64_t n = 0, npad = 0, cap = 0;

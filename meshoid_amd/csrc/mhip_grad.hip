@@ -173,6 +173,7 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     void *fs = scratch(ctx, (size_t)K * ix->npad * fb, &rc);
     if (!fs) return rc;
     const dim3 gp((unsigned)((ix->npad + 255) / 256));
+    ProfRegion prp(ctx, "apply_permute_fields");
     if (f_f64)
         hipLaunchKernelGGL(k_permute_fields<double>, gp, dim3(256), 0, ctx->stream, (const double *)f,
                            nf_rows, K, ix->perm, ix->n, ix->npad, (double *)fs);
@@ -204,6 +205,8 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     a.out1 = out1;
     a.out2 = out2;
     a.status = status;
+    prp.end();
+    ProfRegion pra(ctx, order == 1 ? "apply_D1" : "apply_D2");
     const dim3 g((unsigned)((nl->nq + 255) / 256));
     if (f_f64)
         dispatch_apply<double>(a, ix->dim, order, g, ctx->stream);

@@ -409,6 +409,7 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     float *part = (float *)(s + off_bb), *bb = part + nbb * 8;
     const int nb = (int)std::min<int64_t>(nbb, (n + 255) / 256);
     const dim3 g1((unsigned)((n + 255) / 256)), b256(256);
+    ProfRegion pr_bb(ctx, "index_bbox");
     if (pos_f64) {
         hipLaunchKernelGGL(k_bbox_partial<double>, dim3(nb), b256, 0, ctx->stream, (const double *)pos,
                            n, dim, part);
@@ -417,6 +418,8 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
                            n, dim, part);
     }
     hipLaunchKernelGGL(k_bbox_final, dim3(1), b256, 0, ctx->stream, part, nb, dim, (float)box, bb);
+    pr_bb.end();
+    ProfRegion pr_key(ctx, "index_morton_sort");
     if (pos_f64)
         hipLaunchKernelGGL(k_morton<double>, g1, b256, 0, ctx->stream, (const double *)pos, n, dim, bb,
                            k0, v0);
@@ -426,6 +429,8 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     uint64_t *ks;
     uint32_t *vs;
     if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, 21 * dim, &ks, &vs))) return rc;
+    pr_key.end();
+    ProfRegion pr_g(ctx, "index_reorder_boxes");
     const dim3 gp((unsigned)((npad + 255) / 256));
     if (pos_f64)
         hipLaunchKernelGGL(k_gather_sorted<double>, gp, b256, 0, ctx->stream, (const double *)pos, n,
@@ -441,6 +446,7 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
         hipLaunchKernelGGL(k_node_boxes, dim3((unsigned)((lev_n[l] + 255) / 256)), b256, 0,
                            ctx->stream, ix->nodes + lev_off[l - 1] * 8, lev_n[l - 1],
                            ix->nodes + lev_off[l] * 8, lev_n[l]);
+    pr_g.end();
     if ((rc = check_hip(ctx, hipGetLastError(), "index build launch"))) return rc;
     // bbox + nan flag back to the host: periodic range and NaN checks
     float hb[8];

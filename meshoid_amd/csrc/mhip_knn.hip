@@ -545,6 +545,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     const bool masked = qidx != nullptr;
     nl->masked = masked;
     if (masked) {
+        ProfRegion pr(ctx, "knn_query_setup");
         size_t cw = radix_counts_words(nq);
         char *s = (char *)scratch(ctx, nq * 24 + cw * 4, &rc);
         if (!s) return rc;
@@ -604,6 +605,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     const dim3 g((unsigned)((nq + KNN_BLOCK - 1) / KNN_BLOCK));
     hipStream_t s = ctx->stream;
     const int need = k + 1;
+    ProfRegion pr1(ctx, "knn_pass1");
     if (need <= 5)
         launch_pass1<5>(a, per, p64, g, s);
     else if (need <= 9)
@@ -620,6 +622,8 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
         launch_pass1<101>(a, per, p64, g, s);
     else
         launch_pass1<129>(a, per, p64, g, s);
+    pr1.end();
+    ProfRegion pr2(ctx, "knn_pass2");
     if (per) {
         if (p64)
             hipLaunchKernelGGL((k_knn_pass2<true, true>), g, dim3(KNN_BLOCK), 0, s, a);
@@ -631,6 +635,8 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
         else
             hipLaunchKernelGGL((k_knn_pass2<false, false>), g, dim3(KNN_BLOCK), 0, s, a);
     }
+    pr2.end();
+    ProfRegion pr3(ctx, "knn_resolve");
     const dim3 gr((unsigned)((nq + 63) / 64));
     if (per) {
         if (p64)
@@ -647,6 +653,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
             hipLaunchKernelGGL((k_knn_resolve<false, false, float>), gr, dim3(64), 0, s, a,
                                (const float *)ix->pos_in);
     }
+    pr3.end();
     if (status_out)
         hipLaunchKernelGGL(k_copy_status, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s,
                            nl->status, a.qrow, ix->perm, nq, status_out);
@@ -682,6 +689,7 @@ int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *d
     const mhip_index *ix = nl->index;
     if (!ix || nl->nq < 1) return fail(ctx, -1, "neighbour list is empty");
     if (!nl->has_d2) return fail(ctx, -1, "neighbour list was built without distances");
+    ProfRegion pr(ctx, "nlist_export");
     hipLaunchKernelGGL(k_export, dim3((unsigned)((nl->nq + 255) / 256)), dim3(256), 0, ctx->stream,
                        nl->idx, nl->d2, nl->masked ? nl->qrow : nullptr, ix->perm, nl->nq, nl->k,
                        ngb_out, dist_out);

@@ -38,6 +38,24 @@ def context(device=None) -> ctypes.c_void_p:
     return ctx
 
 
+def prof_enable(on=True, device=None):
+    """Record HIP events around every library launch region."""
+    ctx = context(device)
+    raise_for(lib.mhip_prof_enable(ctx, int(bool(on))), ctx)
+
+
+def prof_read(device=None) -> dict:
+    """{region: (launches, total_ms)} since the last read (synchronises)."""
+    ctx = context(device)
+    buf = ctypes.create_string_buffer(1 << 16)
+    raise_for(lib.mhip_prof_read(ctx, buf, len(buf)), ctx)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, n, ms = line.split()
+        out[name] = (int(n), float(ms))
+    return out
+
+
 def ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
