@@ -108,6 +108,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=1 << 21)
     ap.add_argument("--check", type=int, default=1 << 16, help="oracle sample size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stats", action="store_true", help="kNN work counters (slower)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -147,7 +148,7 @@ def main():
         step()
     torch.cuda.synchronize()
     engine.prof_read()
-    engine.prof_enable(True)
+    engine.prof_enable(2 if args.stats else 1)
     log(f"timing {args.steps} steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -156,8 +157,12 @@ def main():
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     prof = engine.prof_read()
-    engine.prof_enable(False)
-    kernels = {name: tot / cnt for name, (cnt, tot) in prof.items()}
+    engine.prof_enable(0)
+    stats = {nm: cnt for nm, (cnt, _) in prof.items() if nm.startswith("stat_")}
+    kernels = {name: tot / cnt for name, (cnt, tot) in prof.items() if not name.startswith("stat_")}
+    if stats:
+        w = max(stats.get("stat_waves", 1), 1)
+        log("per-wave work:", {a[5:]: round(b / w, 2) for a, b in stats.items()})
     log("per-step region ms:", {a: round(b, 3) for a, b in kernels.items()})
     dom = max((r for r in kernels if alg_bytes(r, k, K) > 0), key=lambda r: kernels[r])
     ach = alg_bytes(dom, k, K) * n / (kernels[dom] * 1e-3)

@@ -56,11 +56,13 @@ const char *mhip_last_error(const mhip_ctx *ctx);
 
 /*
  * Kernel timing (no counterpart in the reference, whose notebooks use %time).
- * While enabled, every launch region records HIP events on the context
- * stream.  mhip_prof_read synchronises, writes one line per region,
- * "name launches total_ms\n", into buf and clears the records.
+ * level 1: every launch region records HIP events on the context stream;
+ * level 2: also per-wavefront work counters of the kNN passes; 0: off.
+ * mhip_prof_read synchronises, writes one line per region,
+ * "name launches total_ms\n" (counters: "stat_name count 0\n"), into buf and
+ * clears the records.
  */
-int mhip_prof_enable(mhip_ctx *ctx, int on);
+int mhip_prof_enable(mhip_ctx *ctx, int level);
 int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen);
 
 /*

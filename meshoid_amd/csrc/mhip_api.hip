@@ -75,6 +75,7 @@ int mhip_ctx_destroy(mhip_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->stats) (void)hipFree(ctx->stats);
     for (auto &r : ctx->recs) {
         (void)hipEventDestroy(r.start);
         (void)hipEventDestroy(r.stop);
@@ -84,9 +85,17 @@ int mhip_ctx_destroy(mhip_ctx *ctx) {
     return MHIP_OK;
 }
 
-int mhip_prof_enable(mhip_ctx *ctx, int on) {
+int mhip_prof_enable(mhip_ctx *ctx, int level) {
     if (!ctx) return MHIP_ERR_VALUE;
-    ctx->prof = on ? 1 : 0;
+    ctx->prof = level < 0 ? 0 : level;
+    if (ctx->prof >= 2 && !ctx->stats) {
+        int rc = check_hip(ctx, hipMalloc((void **)&ctx->stats, 16 * sizeof(unsigned long long)),
+                           "hipMalloc (stats)");
+        if (rc) return rc;
+        rc = check_hip(ctx, hipMemsetAsync(ctx->stats, 0, 16 * sizeof(unsigned long long), ctx->stream),
+                       "stats clear");
+        if (rc) return rc;
+    }
     return MHIP_OK;
 }
 
@@ -119,6 +128,19 @@ int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen) {
     for (size_t i = 0; i < names.size(); i++) {
         snprintf(line, sizeof line, "%s %ld %.6f\n", names[i].c_str(), cnt[i], tot[i]);
         out += line;
+    }
+    if (ctx->stats) {
+        static const char *stat_names[STAT_COUNT] = {"stat_p1_leaves", "stat_p1_inserts", "stat_p1_nodes",
+                                                     "stat_p2_leaves", "stat_p2_nodes", "stat_waves"};
+        unsigned long long hs[16];
+        rc = check_hip(ctx, hipMemcpy(hs, ctx->stats, sizeof hs, hipMemcpyDeviceToHost), "stats read");
+        if (rc) return rc;
+        for (int i = 0; i < STAT_COUNT; i++) {
+            snprintf(line, sizeof line, "%s %llu 0\n", stat_names[i], hs[i]);
+            out += line;
+        }
+        rc = check_hip(ctx, hipMemset(ctx->stats, 0, sizeof hs), "stats clear");
+        if (rc) return rc;
     }
     if ((int64_t)out.size() + 1 > buflen) return fail(ctx, MHIP_ERR_VALUE, "profile buffer too small");
     memcpy(buf, out.c_str(), out.size() + 1);

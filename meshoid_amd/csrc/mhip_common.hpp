@@ -22,7 +22,7 @@ namespace mhip {
 constexpr int WAVE = 64;
 constexpr int LEAF = 16;    // particles per leaf of the implicit tree
 constexpr int FANOUT = 8;   // children per internal node
-constexpr int MAX_LEVELS = 16;
+constexpr int MAX_DEPTH = 10;  // complete 8-ary tree: up to 8^10 leaves
 
 // status bits written per query row
 enum : int32_t {
@@ -118,7 +118,21 @@ struct mhip_ctx {
     int prof = 0;
     std::vector<mhip_prof_rec> recs;
     std::vector<hipEvent_t> event_pool;
+    // work counters (prof >= 2): device u64[16], summed per wavefront
+    unsigned long long *stats = nullptr;
 };
+
+namespace mhip {
+enum : int {
+    STAT_P1_LEAVES = 0,   // leaves scanned by the kNN main pass (sum over lanes)
+    STAT_P1_INSERTS,      // insertion steps executed per wavefront (sum over waves)
+    STAT_P1_NODES,        // tree nodes tested (sum over lanes)
+    STAT_P2_LEAVES,       // leaves scanned in pass 2
+    STAT_P2_NODES,        // tree nodes popped in pass 2
+    STAT_WAVES,           // wavefronts of pass 1
+    STAT_COUNT
+};
+}
 
 namespace mhip {
 // records start/stop events around a launch region when profiling is on
@@ -163,10 +177,12 @@ struct mhip_index {
     uint32_t *perm = nullptr;                           // sorted -> original
     uint32_t *iperm = nullptr;                          // original -> sorted
     const void *pos_in = nullptr;                       // caller's positions (for fp64 resolution)
-    // implicit tree: level 0 = leaves; node boxes as 8 floats (lo xyz, hi xyz, pad)
-    int nlev = 0;
-    int64_t lev_n[mhip::MAX_LEVELS] = {};
-    int64_t lev_off[mhip::MAX_LEVELS] = {};
+    // complete 8-ary tree in heap numbering: node x has children 8x+1..8x+8,
+    // leaves are nodes first_leaf .. first_leaf + 8^depth - 1, leaf j holds
+    // sorted particles [LEAF*j, LEAF*j + LEAF); leaves j >= nleaf are empty
+    // (box lo = +inf, hi = -inf).  Boxes: 8 floats (lo xyz, hi xyz, pad).
+    int depth = 0;
+    int64_t nleaf = 0, first_leaf = 0, nnodes = 0;
     float *nodes = nullptr;
     int64_t node_cap = 0;
     float bbox[6] = {};

@@ -297,51 +297,44 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const PT *__restrict__ po
 
 __global__ __launch_bounds__(256) void k_leaf_boxes(const float *__restrict__ sx, const float *__restrict__ sy,
                                                    const float *__restrict__ sz, int64_t n, int64_t nleaf,
-                                                   float *__restrict__ nodes) {
+                                                   int64_t nslots, float *__restrict__ leaves) {
     int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (L >= nleaf) return;
+    if (L >= nslots) return;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    int64_t b = L * LEAF, e = min<int64_t>(b + LEAF, n);
-    for (int64_t p = b; p < e; p++) {
-        float v[3] = {sx[p], sy[p], sz[p]};
-        for (int c = 0; c < 3; c++) {
-            lo[c] = fminf(lo[c], v[c]);
-            hi[c] = fmaxf(hi[c], v[c]);
+    if (L < nleaf) {
+        int64_t b = L * LEAF, e = min<int64_t>(b + LEAF, n);
+        for (int64_t p = b; p < e; p++) {
+            float v[3] = {sx[p], sy[p], sz[p]};
+            for (int c = 0; c < 3; c++) {
+                lo[c] = fminf(lo[c], v[c]);
+                hi[c] = fmaxf(hi[c], v[c]);
+            }
         }
     }
-    float *o = nodes + L * 8;
-    o[0] = lo[0];
-    o[1] = lo[1];
-    o[2] = lo[2];
-    o[3] = hi[0];
-    o[4] = hi[1];
-    o[5] = hi[2];
-    o[6] = 0.f;
-    o[7] = 0.f;
+    float4 *o = (float4 *)(leaves + L * 8);
+    o[0] = make_float4(lo[0], lo[1], lo[2], hi[0]);
+    o[1] = make_float4(hi[1], hi[2], 0.f, 0.f);
 }
 
-__global__ __launch_bounds__(256) void k_node_boxes(const float *__restrict__ child, int64_t nchild,
-                                                   float *__restrict__ parent, int64_t nparent) {
-    int64_t P = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (P >= nparent) return;
+// parents x in [x0, x0 + count) of one depth: union of children 8x+1..8x+8
+__global__ __launch_bounds__(256) void k_node_boxes(float *__restrict__ nodes, int64_t x0, int64_t count) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int64_t x = x0 + i;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    int64_t b = P * FANOUT, e = min<int64_t>(b + FANOUT, nchild);
-    for (int64_t c = b; c < e; c++) {
-        const float *s = child + c * 8;
-        for (int a = 0; a < 3; a++) {
-            lo[a] = fminf(lo[a], s[a]);
-            hi[a] = fmaxf(hi[a], s[3 + a]);
-        }
+    for (int c = 1; c <= FANOUT; c++) {
+        const float4 *s = (const float4 *)(nodes + (FANOUT * x + c) * 8);
+        const float4 a = s[0], b = s[1];
+        lo[0] = fminf(lo[0], a.x);
+        lo[1] = fminf(lo[1], a.y);
+        lo[2] = fminf(lo[2], a.z);
+        hi[0] = fmaxf(hi[0], a.w);
+        hi[1] = fmaxf(hi[1], b.x);
+        hi[2] = fmaxf(hi[2], b.y);
     }
-    float *o = parent + P * 8;
-    o[0] = lo[0];
-    o[1] = lo[1];
-    o[2] = lo[2];
-    o[3] = hi[0];
-    o[4] = hi[1];
-    o[5] = hi[2];
-    o[6] = 0.f;
-    o[7] = 0.f;
+    float4 *o = (float4 *)(nodes + x * 8);
+    o[0] = make_float4(lo[0], lo[1], lo[2], hi[0]);
+    o[1] = make_float4(hi[1], hi[2], 0.f, 0.f);
 }
 
 static int grow(mhip_ctx *ctx, void **p, size_t bytes) {
@@ -361,17 +354,16 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     const int64_t npad = (n + LEAF - 1) / LEAF * LEAF;
     const int64_t nleaf = npad / LEAF;
     int rc = 0;
-    // level sizes
-    int nlev = 0;
-    int64_t lev_n[MAX_LEVELS], lev_off[MAX_LEVELS], tot = 0;
-    for (int64_t m = nleaf;; m = (m + FANOUT - 1) / FANOUT) {
-        if (nlev == MAX_LEVELS) return fail(ctx, -4, "tree too deep");
-        lev_n[nlev] = m;
-        lev_off[nlev] = tot;
-        tot += m;
-        nlev++;
-        if (m == 1) break;
+    // complete 8-ary tree over the leaves
+    int depth = 0;
+    int64_t nslots = 1;
+    while (nslots < nleaf) {
+        nslots *= FANOUT;
+        depth++;
     }
+    if (depth > MAX_DEPTH) return fail(ctx, -4, "tree too deep");
+    const int64_t first_leaf = (nslots - 1) / (FANOUT - 1);
+    const int64_t tot = first_leaf + nslots;
     if (ix->cap < npad || (pos_f64 && !ix->lx)) {
         void **arrs[6] = {(void **)&ix->sx, (void **)&ix->sy, (void **)&ix->sz,
                           (void **)&ix->lx, (void **)&ix->ly, (void **)&ix->lz};
@@ -391,11 +383,10 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     ix->box = box;
     ix->pos_f64 = pos_f64;
     ix->pos_in = pos;
-    ix->nlev = nlev;
-    for (int l = 0; l < nlev; l++) {
-        ix->lev_n[l] = lev_n[l];
-        ix->lev_off[l] = lev_off[l];
-    }
+    ix->depth = depth;
+    ix->nleaf = nleaf;
+    ix->first_leaf = first_leaf;
+    ix->nnodes = tot;
     // scratch layout: keys x2, vals x2, counts, bbox partials
     const size_t cw = radix_counts_words(n);
     const int nbb = 1024;
@@ -440,12 +431,17 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
         hipLaunchKernelGGL(k_gather_sorted<float>, gp, b256, 0, ctx->stream, (const float *)pos, n,
                            npad, dim, vs, ix->sx, ix->sy, ix->sz, (float *)nullptr, (float *)nullptr,
                            (float *)nullptr, ix->perm, ix->iperm);
-    hipLaunchKernelGGL(k_leaf_boxes, dim3((unsigned)((nleaf + 255) / 256)), b256, 0, ctx->stream,
-                       ix->sx, ix->sy, ix->sz, n, nleaf, ix->nodes);
-    for (int l = 1; l < nlev; l++)
-        hipLaunchKernelGGL(k_node_boxes, dim3((unsigned)((lev_n[l] + 255) / 256)), b256, 0,
-                           ctx->stream, ix->nodes + lev_off[l - 1] * 8, lev_n[l - 1],
-                           ix->nodes + lev_off[l] * 8, lev_n[l]);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3((unsigned)((nslots + 255) / 256)), b256, 0, ctx->stream,
+                       ix->sx, ix->sy, ix->sz, n, nleaf, nslots, ix->nodes + first_leaf * 8);
+    for (int d = depth - 1; d >= 0; d--) {
+        int64_t x0 = 0, cnt = 1;
+        for (int e = 0; e < d; e++) {
+            x0 += cnt;
+            cnt *= FANOUT;
+        }
+        hipLaunchKernelGGL(k_node_boxes, dim3((unsigned)((cnt + 255) / 256)), b256, 0, ctx->stream,
+                           ix->nodes, x0, cnt);
+    }
     pr_g.end();
     if ((rc = check_hip(ctx, hipGetLastError(), "index build launch"))) return rc;
     // bbox + nan flag back to the host: periodic range and NaN checks

@@ -6,39 +6,49 @@
 //                                                src/meshoid/kernel_density.py:14-57
 //   density = des_ngb*m/(volnorm*h^dim)          src/meshoid/Meshoid.py:169-170
 //
-// Design (one wavefront = 64 consecutive queries in Morton order):
-//   pass 1  wave-uniform traversal of the implicit tree (children near-to-far
-//           through a per-wave LDS stack); every lane keeps the k+1 smallest
-//           float32 squared distances in registers, inserted with v_med3 (one
-//           instruction per slot).  The epilogue solves the smoothing length on
-//           those k distances (safeguarded Newton, compensated float32 sums) and
-//           flags queries whose k-th and (k+1)-th distances are closer than
-//           float32 rounding can order.
-//   pass 2  same traversal with the now exact radius tau_i = k-th distance:
-//           collects exactly the k members (ties at tau capped by the count of
-//           members strictly inside) into a column-major list.
-//   resolve for flagged queries only: the members in the rounding band around
-//           tau are re-ranked with float64 distances computed from the
-//           caller's positions, so neighbour sets equal the float64 kNN except
-//           for float64 ties.
+// Design (one lane per query; 64 consecutive queries in Morton order per
+// wavefront; the tree is a complete 8-ary tree in heap numbering):
+//   main    (1) priming: the leaves holding the wave's own queries; every
+//               lane keeps the k+1 smallest float32 squared distances in
+//               VGPRs, sorted, inserted with v_med3 (one instruction per slot);
+//           (2) wave-uniform depth-first traversal (children near-to-far via
+//               a per-wave LDS stack); node boxes and leaf candidates are read
+//               with scalar loads.  A candidate under its lane's current
+//               (k+1)-th distance goes to that lane's LDS queue; the queues
+//               drain into the sorted lists in batches, so the divergent
+//               insertion runs once per batch, not per candidate.  Leaves where
+//               some lane queued a candidate are recorded;
+//           (3) epilogue: smoothing length on the k distances (safeguarded
+//               Newton, compensated float32 sums), density, and the exact
+//               member set {d2 <= tau} collected from the primed and recorded
+//               leaves only (no second traversal).  Queries whose k-th and
+//               (k+1)-th distances are closer than float32 rounding can order
+//               are flagged.
+//   pass2   fallback for waves whose leaf record overflowed: per-lane
+//           traversal with the final radius, collecting the members.
+//   resolve for flagged queries only: members in the rounding band around tau
+//           are re-ranked with float64 distances from the caller's positions,
+//           so neighbour sets equal the float64 kNN except for float64 ties.
 #include "mhip_internal.hpp"
 
 namespace mhip {
 
 constexpr int KNN_BLOCK = 256;
 constexpr int KNN_WAVES = KNN_BLOCK / WAVE;
-constexpr int STACK = FANOUT * MAX_LEVELS;
+constexpr int QCAP = 32;                     // per-lane pending-candidate queue
+constexpr int STACK = FANOUT * (MAX_DEPTH + 1);  // per-wave traversal stack
+constexpr int WLCAP = 384;                   // per-wave record of candidate leaves
 constexpr float U32 = 5.9604645e-8f;         // 2^-24
 constexpr float AMBIG_REL = 12.0f * U32;     // fp32 vs fp64 ordering ambiguity of d^2
 constexpr float BAND_REL = 16.0f * U32;      // resolver band half-width
 constexpr float PRUNE_REL = 1.0f + 1e-5f;    // box-test safety factor
 constexpr int BAND_CAP = 32;
+constexpr int32_t ST_REDO = 1 << 16;          // internal: members collected by pass2
 
 struct TreeView {
     const float *sx, *sy, *sz, *lx, *ly, *lz;
     const float *nodes;
-    int nlev;
-    int64_t lev_n[MAX_LEVELS], lev_off[MAX_LEVELS];
+    int64_t first_leaf, nleaf;
     float box, half;  // periodic box length and half of it (0: open)
     float slack;      // absolute box-test slack: 0 for float32 inputs, covers
                       // the float32 residuals of float64 inputs otherwise
@@ -100,55 +110,50 @@ MHIP_DEV float axis_gap(float q, float lo, float hi, float box, float slack) {
     return fmaxf(g - slack, 0.f);
 }
 
-// lower bound of the squared distance from q to any particle in box nd
+// lower bound of the squared distance from q to any particle under node x
 template <bool PER>
-MHIP_DEV float box_d2(const TreeView &t, const QPos &q, const float *nd) {
-    float dx = axis_gap<PER>(q.x, nd[0], nd[3], t.box, t.slack);
-    float dy = axis_gap<PER>(q.y, nd[1], nd[4], t.box, t.slack);
-    float dz = axis_gap<PER>(q.z, nd[2], nd[5], t.box, t.slack);
+MHIP_DEV float node_d2(const TreeView &t, const QPos &q, uint32_t x) {
+    const float4 *nb = (const float4 *)(t.nodes + (int64_t)x * 8);
+    const float4 a = nb[0], b = nb[1];  // lo xyz, hi x | hi yz
+    const float dx = axis_gap<PER>(q.x, a.x, a.w, t.box, t.slack);
+    const float dy = axis_gap<PER>(q.y, a.y, b.x, t.box, t.slack);
+    const float dz = axis_gap<PER>(q.z, a.z, b.y, t.box, t.slack);
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
-MHIP_DEV uint32_t node_id(int lev, int64_t i) { return ((uint32_t)lev << 28) | (uint32_t)i; }
+// next node in depth-first order after the subtree of x is finished; false
+// at the end of the traversal (children of p are 8p+1..8p+8)
+MHIP_DEV bool next_after(uint32_t &x) {
+    while (x != 0 && (x & (FANOUT - 1)) == 0) x = (x - 1) >> 3;
+    if (x == 0) return false;
+    x++;
+    return true;
+}
 
-// Wave-uniform depth-first traversal.  thresh() returns this lane's current
-// squared search radius; leaf(L) processes leaf L.  Children of an expanded
-// node are pushed far-to-near relative to the wave's centre query, so the
-// nearest child is popped first.
-template <bool PER, class Thresh, class Leaf>
-MHIP_DEV void traverse(const TreeView &t, const QPos &q, const QPos &centre, uint32_t *stk,
-                       int64_t skip_lo, int64_t skip_hi, Thresh thresh, Leaf leaf) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    int sp = 0;
-    if (lane == 0) stk[0] = node_id(t.nlev - 1, 0);
-    sp = 1;
-    __builtin_amdgcn_wave_barrier();
-    while (sp > 0) {
-        sp--;
-        const uint32_t node = uniform(stk[sp]);
-        const int lev = (int)(node >> 28);
-        const int64_t ix = node & 0x0fffffffu;
-        const float *nd = t.nodes + (t.lev_off[lev] + ix) * 8;
-        const float bd = box_d2<PER>(t, q, nd);
-        if (!__any(bd <= thresh() * PRUNE_REL)) continue;
-        if (lev == 0) {
-            if (ix >= skip_lo && ix <= skip_hi) continue;
-            leaf(ix);
-        } else {
-            const int64_t c0 = ix * FANOUT;
-            const int nc = (int)min<int64_t>(FANOUT, t.lev_n[lev - 1] - c0);
-            // rank children by their box distance to the centre query
-            float cd = INFINITY;
-            if (lane < nc) cd = box_d2<PER>(t, centre, t.nodes + (t.lev_off[lev - 1] + c0 + lane) * 8);
-            int rank = 0;
+// one leaf's candidates of this lane (per-lane gathers, 64-byte aligned)
+template <bool P64>
+MHIP_DEV void load_leaf(const TreeView &t, int64_t leaf, float (&cx)[LEAF], float (&cy)[LEAF],
+                        float (&cz)[LEAF], float (&clx)[LEAF], float (&cly)[LEAF], float (&clz)[LEAF]) {
+    const int64_t base = leaf * LEAF;
 #pragma unroll
-            for (int o = 0; o < FANOUT; o++) {
-                float od = __shfl(cd, o);
-                rank += (o < nc) && (od < cd || (od == cd && o < lane));
-            }
-            if (lane < nc) stk[sp + nc - 1 - rank] = node_id(lev - 1, c0 + lane);
-            __builtin_amdgcn_wave_barrier();
-            sp += nc;
+    for (int v = 0; v < LEAF / 4; v++) {
+        const float4 X = ((const float4 *)(t.sx + base))[v];
+        const float4 Y = ((const float4 *)(t.sy + base))[v];
+        const float4 Z = ((const float4 *)(t.sz + base))[v];
+        cx[4 * v] = X.x, cx[4 * v + 1] = X.y, cx[4 * v + 2] = X.z, cx[4 * v + 3] = X.w;
+        cy[4 * v] = Y.x, cy[4 * v + 1] = Y.y, cy[4 * v + 2] = Y.z, cy[4 * v + 3] = Y.w;
+        cz[4 * v] = Z.x, cz[4 * v + 1] = Z.y, cz[4 * v + 2] = Z.z, cz[4 * v + 3] = Z.w;
+        if (P64) {
+            const float4 LX = ((const float4 *)(t.lx + base))[v];
+            const float4 LY = ((const float4 *)(t.ly + base))[v];
+            const float4 LZ = ((const float4 *)(t.lz + base))[v];
+            clx[4 * v] = LX.x, clx[4 * v + 1] = LX.y, clx[4 * v + 2] = LX.z, clx[4 * v + 3] = LX.w;
+            cly[4 * v] = LY.x, cly[4 * v + 1] = LY.y, cly[4 * v + 2] = LY.z, cly[4 * v + 3] = LY.w;
+            clz[4 * v] = LZ.x, clz[4 * v + 1] = LZ.y, clz[4 * v + 2] = LZ.z, clz[4 * v + 3] = LZ.w;
+        } else {
+            clx[4 * v] = clx[4 * v + 1] = clx[4 * v + 2] = clx[4 * v + 3] = 0.f;
+            cly[4 * v] = cly[4 * v + 1] = cly[4 * v + 2] = cly[4 * v + 3] = 0.f;
+            clz[4 * v] = clz[4 * v + 1] = clz[4 * v + 2] = clz[4 * v + 3] = 0.f;
         }
     }
 }
@@ -158,6 +163,18 @@ MHIP_DEV void insert_sorted(float (&A)[KCAP], float x) {
 #pragma unroll
     for (int j = KCAP - 1; j > 0; --j) A[j] = med3(A[j - 1], x, A[j]);
     A[0] = fminf(A[0], x);
+}
+
+MHIP_DEV int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+MHIP_DEV uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
 }
 
 struct KnnArgs {
@@ -171,20 +188,52 @@ struct KnnArgs {
     const float *seed_h;   // per output row, null = none
     float *h_out, *rho_out, *vol_out;
     int32_t *status_out;
-    // per slot scratch for pass 2
+    // per slot state
     float *tau;
     int32_t *nlt;
     float *h_slot;
     int32_t *st_slot;
     uint32_t *nl_idx;
     float *nl_d2;
+    unsigned long long *stats;  // work counters, may be null
 };
 
-MHIP_DEV int64_t slot_of_lane(const KnnArgs &a, int64_t *first) {
+MHIP_DEV int64_t slot_of_lane(int64_t *first) {
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t wave0 = ((int64_t)b * KNN_WAVES + (threadIdx.x >> 6)) * WAVE;
     *first = wave0;
     return wave0 + (threadIdx.x & (WAVE - 1));
+}
+
+// collects the members {d2 < tau} plus ties d2 == tau up to k of one leaf
+template <bool PER, bool P64>
+MHIP_DEV void collect_leaf(const KnnArgs &a, const QPos &q, int64_t leaf, float tau, int &eq_left,
+                           int &cnt, int64_t slot) {
+    float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
+    load_leaf<P64>(a.t, leaf, cx, cy, cz, clx, cly, clz);
+    const int64_t base = leaf * LEAF;
+#pragma unroll
+    for (int i = 0; i < LEAF; i++) {
+        const float d2 = dist2<PER, P64>(a.t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
+        bool take = d2 < tau;
+        if (d2 == tau && eq_left > 0) {
+            take = true;
+            eq_left--;
+        }
+        if (take && cnt < a.k) {
+            a.nl_idx[(int64_t)cnt * a.nq + slot] = (uint32_t)(base + i);
+            if (a.nl_d2) a.nl_d2[(int64_t)cnt * a.nq + slot] = d2;
+            cnt++;
+        }
+    }
+}
+
+MHIP_DEV void pad_list(const KnnArgs &a, int cnt, uint32_t qp, int64_t slot) {
+    // never leave undefined indices behind: pad with the query itself
+    for (int j = cnt; j < a.k; j++) {
+        a.nl_idx[(int64_t)j * a.nq + slot] = qp;
+        if (a.nl_d2) a.nl_d2[(int64_t)j * a.nq + slot] = INFINITY;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -250,125 +299,257 @@ MHIP_DEV float hsml_solve(const float (&A)[KCAP], int k, int dim, float seed, in
     return h;
 }
 
+// Read-only data read at wave-uniform addresses goes through the constant
+// address space, which the compiler serves with scalar (SMEM) loads into
+// SGPRs: one load per wavefront instead of 64 lanes, and no VGPRs.
+typedef const float __attribute__((address_space(4))) cfloat;
+MHIP_DEV float ldc(const float *p, int64_t i) { return ((cfloat *)p)[i]; }
+
+// one leaf's candidates, wave-uniform address (scalar loads)
+template <bool P64>
+MHIP_DEV void load_leaf_uniform(const float *sx, const float *sy, const float *sz, const float *lx,
+                                const float *ly, const float *lz, int64_t leaf, float (&cx)[LEAF],
+                                float (&cy)[LEAF], float (&cz)[LEAF], float (&clx)[LEAF],
+                                float (&cly)[LEAF], float (&clz)[LEAF]) {
+    const int64_t base = leaf * LEAF;
+#pragma unroll
+    for (int i = 0; i < LEAF; i++) {
+        cx[i] = ldc(sx, base + i);
+        cy[i] = ldc(sy, base + i);
+        cz[i] = ldc(sz, base + i);
+        clx[i] = P64 ? ldc(lx, base + i) : 0.f;
+        cly[i] = P64 ? ldc(ly, base + i) : 0.f;
+        clz[i] = P64 ? ldc(lz, base + i) : 0.f;
+    }
+}
+
+// node_d2 for a wave-uniform node (scalar loads of the box)
+template <bool PER>
+MHIP_DEV float node_d2_u(const TreeView &t, const QPos &q, uint32_t x) {
+    const int64_t o = (int64_t)x * 8;
+    const float lox = ldc(t.nodes, o), loy = ldc(t.nodes, o + 1), loz = ldc(t.nodes, o + 2);
+    const float hix = ldc(t.nodes, o + 3), hiy = ldc(t.nodes, o + 4), hiz = ldc(t.nodes, o + 5);
+    const float dx = axis_gap<PER>(q.x, lox, hix, t.box, t.slack);
+    const float dy = axis_gap<PER>(q.y, loy, hiy, t.box, t.slack);
+    const float dz = axis_gap<PER>(q.z, loz, hiz, t.box, t.slack);
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
 template <int KCAP, bool PER, bool P64>
-__global__ __launch_bounds__(KNN_BLOCK) void k_knn_pass1(KnnArgs a) {
+__global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
+    __shared__ float s_queue[KNN_WAVES][QCAP][WAVE];
     __shared__ uint32_t s_stack[KNN_WAVES][STACK];
+    __shared__ uint32_t s_wl[KNN_WAVES][WLCAP];
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
     int64_t wave0;
-    const int64_t slot = slot_of_lane(a, &wave0);
+    const int64_t slot = slot_of_lane(&wave0);
     if (wave0 >= a.nq) return;
+    const bool live = slot < a.nq;
     const int64_t qs_slot = min<int64_t>(slot, a.nq - 1);
     const uint32_t qp = a.qsorted ? a.qsorted[qs_slot] : (uint32_t)qs_slot;
     const TreeView &t = a.t;
+    const float *__restrict__ sx = t.sx;
+    const float *__restrict__ sy = t.sy;
+    const float *__restrict__ sz = t.sz;
+    const float *__restrict__ lx = t.lx;
+    const float *__restrict__ ly = t.ly;
+    const float *__restrict__ lz = t.lz;
     const QPos q = load_q<P64>(t, qp);
-    QPos c;
-    c.x = __shfl(q.x, 31);
-    c.y = __shfl(q.y, 31);
-    c.z = __shfl(q.z, 31);
-    c.lx = c.ly = c.lz = 0.f;
+    QPos centre;  // the wave's middle query orders the children near-to-far
+    centre.x = __shfl(q.x, 31);
+    centre.y = __shfl(q.y, 31);
+    centre.z = __shfl(q.z, 31);
+    centre.lx = centre.ly = centre.lz = 0.f;
 
+    // sorted k+1 smallest squared distances; slots below e are -1 sentinels
+    // when k + 1 < KCAP, so A[KCAP-2] is the k-th and A[KCAP-1] the (k+1)-th
     float A[KCAP];
     const int e = KCAP - 1 - a.k;
 #pragma unroll
     for (int j = 0; j < KCAP; j++) A[j] = j < e ? -1.0f : INFINITY;
+    float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
+    uint32_t n_ins = 0, n_leaf = 0, n_nodes = 0;
 
-    auto leaf = [&](int64_t L) {
-        const int64_t base = L * LEAF;
-        float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
-#pragma unroll
-        for (int i = 0; i < LEAF; i++) {
-            cx[i] = t.sx[base + i];
-            cy[i] = t.sy[base + i];
-            cz[i] = t.sz[base + i];
-            clx[i] = P64 ? t.lx[base + i] : 0.f;
-            cly[i] = P64 ? t.ly[base + i] : 0.f;
-            clz[i] = P64 ? t.lz[base + i] : 0.f;
-        }
-#pragma unroll
-        for (int i = 0; i < LEAF; i++) {
-            const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
-            const bool ins = d2 < A[KCAP - 1];
-            if (__any(ins)) insert_sorted<KCAP>(A, ins ? d2 : INFINITY);
-        }
-    };
-    // prime with the leaves holding this wave's own queries
+    // (1) priming: the leaves holding this wave's own queries, inserted directly
     const uint32_t p_first = uniform(__shfl(qp, 0));
     const uint32_t p_last = uniform(__shfl(qp, WAVE - 1));
     const int64_t l0 = p_first / LEAF;
     const int64_t l1 = min<int64_t>(p_last / LEAF, l0 + 7);
-    for (int64_t L = l0; L <= l1; L++) leaf(L);
-    traverse<PER>(t, q, c, s_stack[threadIdx.x >> 6], l0, l1, [&]() { return A[KCAP - 1]; }, leaf);
+    for (int64_t L = l0; L <= l1; L++) {
+        load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, L, cx, cy, cz, clx, cly, clz);
+        n_leaf++;
+#pragma unroll
+        for (int i = 0; i < LEAF; i++) {
+            const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
+            const bool ins = d2 < A[KCAP - 1];
+            if (__any(ins)) {
+                insert_sorted<KCAP>(A, ins ? d2 : INFINITY);
+                n_ins++;
+            }
+        }
+    }
 
-    if (slot >= a.nq) return;
+    // (2) wave-uniform depth-first traversal, children near-to-far through a
+    // per-wave LDS stack.  Candidates under a lane's current (k+1)-th distance
+    // go to its LDS queue; the queues drain into the sorted lists in batches.
+    // Leaves where some lane queued a candidate are recorded for (3).
+    int qn = 0;
+    auto drain = [&]() {
+        const int m = wave_max(qn);
+        for (int i = 0; i < m; i++) {
+            const float v = i < qn ? s_queue[w][i][lane] : INFINITY;
+            insert_sorted<KCAP>(A, v);
+        }
+        n_ins += m;
+        qn = 0;
+    };
+    uint32_t *stk = s_stack[w];
+    uint32_t nwl = 0;  // recorded leaves (uniform)
+    const uint32_t first_leaf = (uint32_t)t.first_leaf;
+    if (lane == 0) stk[0] = 0u;
+    int sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        sp--;
+        const uint32_t x = uniform(stk[sp]);
+        n_nodes++;
+        if (!__any(node_d2_u<PER>(t, q, x) <= A[KCAP - 1] * PRUNE_REL)) continue;
+        if (x >= first_leaf) {
+            const int64_t li = x - first_leaf;
+            // skip primed and empty leaves (empty ones have +inf boxes, which
+            // pass while some list is not full)
+            if ((li >= l0 && li <= l1) || li >= t.nleaf) continue;
+            n_leaf++;
+            load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, li, cx, cy, cz, clx, cly, clz);
+            const int before = qn;
+#pragma unroll
+            for (int i = 0; i < LEAF; i++) {
+                const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
+                if (d2 < A[KCAP - 1]) {
+                    s_queue[w][qn][lane] = d2;
+                    qn++;
+                }
+            }
+            if (__any(qn > before)) {
+                if (nwl < WLCAP && lane == 0) s_wl[w][nwl] = (uint32_t)li;
+                nwl++;
+            }
+            if (__any(qn > QCAP - LEAF)) drain();
+        } else {
+            // children 8x+1..8x+8, pushed far-to-near so the nearest pops first
+            const uint32_t c0 = FANOUT * x + 1;
+            float cd = INFINITY;
+            if (lane < FANOUT) cd = node_d2<PER>(t, centre, c0 + lane);
+            int rank = 0;
+#pragma unroll
+            for (int o = 0; o < FANOUT; o++) {
+                const float od = __shfl(cd, o);
+                rank += od < cd || (od == cd && o < lane);
+            }
+            if (lane < FANOUT) stk[sp + FANOUT - 1 - rank] = c0 + lane;
+            __builtin_amdgcn_wave_barrier();
+            sp += FANOUT;
+        }
+    }
+    drain();
+    __builtin_amdgcn_wave_barrier();
+    if (a.stats && lane == 0) {
+        atomicAdd(a.stats + STAT_P1_LEAVES, (unsigned long long)n_leaf);
+        atomicAdd(a.stats + STAT_P1_NODES, (unsigned long long)n_nodes);
+        atomicAdd(a.stats + STAT_P1_INSERTS, (unsigned long long)n_ins);
+        atomicAdd(a.stats + STAT_P2_LEAVES, (unsigned long long)nwl);
+        atomicAdd(a.stats + STAT_WAVES, 1ull);
+    }
+
+    // (3) epilogue
     int32_t st = 0;
     const float tau = A[KCAP - 2], tau1 = A[KCAP - 1];
     if (tau1 <= tau * (1.0f + AMBIG_REL)) st |= ST_AMBIGUOUS;
     int nlt = 0;
 #pragma unroll
     for (int j = 0; j < KCAP - 1; j++) nlt += (A[j] >= 0.f && A[j] < tau);
-    const uint32_t row = a.qrow ? a.qrow[slot] : a.perm[slot];
+    const uint32_t row = a.qrow ? a.qrow[qs_slot] : a.perm[qs_slot];
     const float seed = a.seed_h ? a.seed_h[row] : 0.f;
     const float h = hsml_solve<KCAP>(A, a.k, a.dim, seed, st);
-    a.tau[slot] = tau;
-    a.nlt[slot] = nlt;
-    a.h_slot[slot] = h;
-    a.st_slot[slot] = st;
-    const double vn = a.dim == 3 ? 4.18879020478639098 : (a.dim == 2 ? 3.14159265358979324 : 2.0);
-    const double m = a.m_rows ? a.m_rows[row] : 1.0;
-    const double hd = (double)h, hp = a.dim == 3 ? hd * hd * hd : (a.dim == 2 ? hd * hd : hd);
-    const double rho = (double)a.k * m / (vn * hp);
-    if (a.h_out) a.h_out[row] = h;
-    if (a.rho_out) a.rho_out[row] = (float)rho;
-    if (a.vol_out) a.vol_out[row] = (float)(m / rho);
+    if (live) {
+        a.tau[slot] = tau;
+        a.nlt[slot] = nlt;
+        a.h_slot[slot] = h;
+        const double vn = a.dim == 3 ? 4.18879020478639098 : (a.dim == 2 ? 3.14159265358979324 : 2.0);
+        const double m = a.m_rows ? a.m_rows[row] : 1.0;
+        const double hd = (double)h, hp = a.dim == 3 ? hd * hd * hd : (a.dim == 2 ? hd * hd : hd);
+        const double rho = (double)a.k * m / (vn * hp);
+        if (a.h_out) a.h_out[row] = h;
+        if (a.rho_out) a.rho_out[row] = (float)rho;
+        if (a.vol_out) a.vol_out[row] = (float)(m / rho);
+    }
+    // members: a candidate at or below tau was inserted while priming or
+    // passed its lane's queue filter when its leaf was scanned, so it lies in
+    // a primed or recorded leaf; leaves no final ball reaches are skipped
+    if (nwl <= WLCAP) {
+        int cnt = 0, eq_left = a.k - nlt;
+        const uint32_t total = (uint32_t)(l1 - l0 + 1) + nwl;
+        for (uint32_t r = 0; r < total; r++) {
+            const int64_t li = r <= (uint32_t)(l1 - l0) ? l0 + r : (int64_t)uniform(s_wl[w][r - (l1 - l0 + 1)]);
+            if (!__any(node_d2_u<PER>(t, q, first_leaf + (uint32_t)li) <= tau * PRUNE_REL)) continue;
+            load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, li, cx, cy, cz, clx, cly, clz);
+#pragma unroll
+            for (int i = 0; i < LEAF; i++) {
+                const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
+                bool take = d2 < tau;
+                if (d2 == tau && eq_left > 0) {
+                    take = true;
+                    eq_left--;
+                }
+                if (take && live && cnt < a.k) {
+                    a.nl_idx[(int64_t)cnt * a.nq + slot] = (uint32_t)(li * LEAF + i);
+                    if (a.nl_d2) a.nl_d2[(int64_t)cnt * a.nq + slot] = d2;
+                    cnt++;
+                }
+            }
+        }
+        if (live && cnt != a.k) {
+            st |= ST_COUNT;
+            pad_list(a, cnt, qp, slot);
+        }
+    } else {
+        st |= ST_REDO;
+    }
+    if (live) a.st_slot[slot] = st;
 }
 
+// members for lanes whose leaf record overflowed: traversal with radius tau
 template <bool PER, bool P64>
-__global__ __launch_bounds__(KNN_BLOCK) void k_knn_pass2(KnnArgs a) {
-    __shared__ uint32_t s_stack[KNN_WAVES][STACK];
-    int64_t wave0;
-    const int64_t slot = slot_of_lane(a, &wave0);
-    if (wave0 >= a.nq) return;
-    const bool live = slot < a.nq;
-    const int64_t qs_slot = min<int64_t>(slot, a.nq - 1);
-    const uint32_t qp = a.qsorted ? a.qsorted[qs_slot] : (uint32_t)qs_slot;
+__global__ __launch_bounds__(256) void k_knn_pass2(KnnArgs a) {
+    const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (slot >= a.nq) return;
+    int32_t st = a.st_slot[slot];
+    if (!(st & ST_REDO)) return;
     const TreeView &t = a.t;
+    const uint32_t qp = a.qsorted ? a.qsorted[slot] : (uint32_t)slot;
     const QPos q = load_q<P64>(t, qp);
-    QPos c;
-    c.x = __shfl(q.x, 31);
-    c.y = __shfl(q.y, 31);
-    c.z = __shfl(q.z, 31);
-    c.lx = c.ly = c.lz = 0.f;
-    const float tau = a.tau[qs_slot];
-    int eq_left = a.k - a.nlt[qs_slot];
-    int cnt = 0;
-    const int64_t nq = a.nq;
-    auto leaf = [&](int64_t L) {
-        const int64_t base = L * LEAF;
-#pragma unroll 4
-        for (int i = 0; i < LEAF; i++) {
-            const float d2 = dist2<PER, P64>(t, q, t.sx[base + i], t.sy[base + i], t.sz[base + i],
-                                             P64 ? t.lx[base + i] : 0.f, P64 ? t.ly[base + i] : 0.f,
-                                             P64 ? t.lz[base + i] : 0.f);
-            bool take = d2 < tau;
-            if (d2 == tau && eq_left > 0) {
-                take = true;
-                eq_left--;
+    const float tau = a.tau[slot];
+    int eq_left = a.k - a.nlt[slot], cnt = 0;
+    const uint32_t first_leaf = (uint32_t)t.first_leaf;
+    uint32_t x = 0;
+    bool more = true;
+    while (more) {
+        if (node_d2<PER>(t, q, x) <= tau * PRUNE_REL) {
+            if (x < first_leaf) {
+                x = FANOUT * x + 1;
+                continue;
             }
-            if (take && live && cnt < a.k) {
-                a.nl_idx[(int64_t)cnt * nq + slot] = (uint32_t)(base + i);
-                if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = d2;
-                cnt++;
-            }
+            if (x - first_leaf < t.nleaf)
+                collect_leaf<PER, P64>(a, q, x - first_leaf, tau, eq_left, cnt, slot);
         }
-    };
-    traverse<PER>(t, q, c, s_stack[threadIdx.x >> 6], 1, 0, [&]() { return tau; }, leaf);
-    if (live && cnt != a.k) {
-        // never leave undefined indices behind: pad with the query itself
-        a.st_slot[slot] |= ST_COUNT;
-        for (int j = cnt; j < a.k; j++) {
-            a.nl_idx[(int64_t)j * nq + slot] = qp;
-            if (a.nl_d2) a.nl_d2[(int64_t)j * nq + slot] = INFINITY;
-        }
+        more = next_after(x);
     }
+    st &= ~ST_REDO;
+    if (cnt != a.k) {
+        st |= ST_COUNT;
+        pad_list(a, cnt, qp, slot);
+    }
+    a.st_slot[slot] = st;
 }
 
 // float64 squared distance between original particles i and j
@@ -400,49 +581,46 @@ __global__ __launch_bounds__(64) void k_knn_resolve(KnnArgs a, const PT *__restr
     uint32_t bi[BAND_CAP];
     int nb = 0, cnt = 0;
     int32_t st = a.st_slot[slot];
-    uint32_t stk[STACK];
-    int sp = 0;
-    stk[sp++] = node_id(t.nlev - 1, 0);
-    while (sp > 0) {
-        const uint32_t node = stk[--sp];
-        const int lev = (int)(node >> 28);
-        const int64_t ix = node & 0x0fffffffu;
-        const float *nd = t.nodes + (t.lev_off[lev] + ix) * 8;
-        if (box_d2<PER>(t, q, nd) > band_hi * PRUNE_REL) continue;
-        if (lev > 0) {
-            const int64_t c0 = ix * FANOUT;
-            const int nc = (int)min<int64_t>(FANOUT, t.lev_n[lev - 1] - c0);
-            for (int i = nc - 1; i >= 0; i--) stk[sp++] = node_id(lev - 1, c0 + i);
-            continue;
-        }
-        for (int i = 0; i < LEAF; i++) {
-            const int64_t p = ix * LEAF + i;
-            const float d2 = dist2<PER, P64>(t, q, t.sx[p], t.sy[p], t.sz[p], P64 ? t.lx[p] : 0.f,
-                                             P64 ? t.ly[p] : 0.f, P64 ? t.lz[p] : 0.f);
-            if (d2 < band_lo) {
-                if (cnt < a.k) {
-                    a.nl_idx[(int64_t)cnt * nq + slot] = (uint32_t)p;
-                    if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = d2;
-                }
-                cnt++;
-            } else if (d2 <= band_hi && p < a.n) {
-                const double e = exact_d2<PT, PER>(pos_in, a.dim, qo, a.perm[p], (double)t.box);
-                if (nb < BAND_CAP) {
-                    bd[nb] = e;
-                    bi[nb] = (uint32_t)p;
-                    nb++;
-                } else {
-                    st |= ST_BAND_OVERFLOW;
-                    int w = 0;  // replace the largest entry if this one is smaller
-                    for (int m = 1; m < BAND_CAP; m++)
-                        if (bd[m] > bd[w]) w = m;
-                    if (e < bd[w]) {
-                        bd[w] = e;
-                        bi[w] = (uint32_t)p;
+    const uint32_t first_leaf = (uint32_t)t.first_leaf;
+    uint32_t x = 0;
+    bool more = true;
+    while (more) {
+        if (node_d2<PER>(t, q, x) <= band_hi * PRUNE_REL) {
+            if (x < first_leaf) {
+                x = FANOUT * x + 1;
+                continue;
+            }
+            const int64_t base = (int64_t)(x - first_leaf) * LEAF;
+            for (int i = 0; i < LEAF && x - first_leaf < t.nleaf; i++) {
+                const int64_t p = base + i;
+                const float d2 = dist2<PER, P64>(t, q, t.sx[p], t.sy[p], t.sz[p], P64 ? t.lx[p] : 0.f,
+                                                 P64 ? t.ly[p] : 0.f, P64 ? t.lz[p] : 0.f);
+                if (d2 < band_lo) {
+                    if (cnt < a.k) {
+                        a.nl_idx[(int64_t)cnt * nq + slot] = (uint32_t)p;
+                        if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = d2;
+                    }
+                    cnt++;
+                } else if (d2 <= band_hi && p < a.n) {
+                    const double e = exact_d2<PT, PER>(pos_in, a.dim, qo, a.perm[p], (double)t.box);
+                    if (nb < BAND_CAP) {
+                        bd[nb] = e;
+                        bi[nb] = (uint32_t)p;
+                        nb++;
+                    } else {
+                        st |= ST_BAND_OVERFLOW;
+                        int wmax = 0;  // replace the largest entry if this one is smaller
+                        for (int m = 1; m < BAND_CAP; m++)
+                            if (bd[m] > bd[wmax]) wmax = m;
+                        if (e < bd[wmax]) {
+                            bd[wmax] = e;
+                            bi[wmax] = (uint32_t)p;
+                        }
                     }
                 }
             }
         }
+        more = next_after(x);
     }
     // order the band by (exact d^2, original index) and take what is missing
     for (int i = 1; i < nb; i++) {
@@ -463,10 +641,7 @@ __global__ __launch_bounds__(64) void k_knn_resolve(KnnArgs a, const PT *__restr
     }
     if (cnt != a.k) {
         st |= ST_COUNT;
-        for (int j = cnt; j < a.k; j++) {
-            a.nl_idx[(int64_t)j * nq + slot] = qp;
-            if (a.nl_d2) a.nl_d2[(int64_t)j * nq + slot] = INFINITY;
-        }
+        pad_list(a, min(cnt, a.k), qp, slot);
     } else {
         st &= ~ST_COUNT;
     }
@@ -498,24 +673,24 @@ __global__ void k_copy_status(const int32_t *__restrict__ st_slot, const uint32_
 }
 
 static int nl_grow(mhip_ctx *ctx, void **p, size_t bytes) {
-    if (*p) hipFree(*p);
+    if (*p) (void)hipFree(*p);
     *p = nullptr;
     if (!bytes) return 0;
     return check_hip(ctx, hipMalloc(p, bytes), "hipMalloc (neighbour list)");
 }
 
 template <int KCAP>
-static void launch_pass1(const KnnArgs &a, bool per, bool p64, dim3 g, hipStream_t s) {
+static void launch_main(const KnnArgs &a, bool per, bool p64, dim3 g, hipStream_t s) {
     if (per) {
         if (p64)
-            hipLaunchKernelGGL((k_knn_pass1<KCAP, true, true>), g, dim3(KNN_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((k_knn_main<KCAP, true, true>), g, dim3(KNN_BLOCK), 0, s, a);
         else
-            hipLaunchKernelGGL((k_knn_pass1<KCAP, true, false>), g, dim3(KNN_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((k_knn_main<KCAP, true, false>), g, dim3(KNN_BLOCK), 0, s, a);
     } else {
         if (p64)
-            hipLaunchKernelGGL((k_knn_pass1<KCAP, false, true>), g, dim3(KNN_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((k_knn_main<KCAP, false, true>), g, dim3(KNN_BLOCK), 0, s, a);
         else
-            hipLaunchKernelGGL((k_knn_pass1<KCAP, false, false>), g, dim3(KNN_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((k_knn_main<KCAP, false, false>), g, dim3(KNN_BLOCK), 0, s, a);
     }
 }
 
@@ -536,6 +711,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
         if ((rc = nl_grow(ctx, (void **)&nl->idx, (size_t)nq * k * 4))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->d2, (size_t)nq * k * 4))) return rc;
         nl->cap = nq;
+        nl->k = k;
     }
     nl->nq = nq;
     nl->k = k;
@@ -569,11 +745,8 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     a.t.ly = ix->ly;
     a.t.lz = ix->lz;
     a.t.nodes = ix->nodes;
-    a.t.nlev = ix->nlev;
-    for (int l = 0; l < MAX_LEVELS; l++) {
-        a.t.lev_n[l] = ix->lev_n[l];
-        a.t.lev_off[l] = ix->lev_off[l];
-    }
+    a.t.first_leaf = ix->first_leaf;
+    a.t.nleaf = ix->nleaf;
     const bool per = ix->box > 0;
     a.t.box = per ? (float)ix->box : 0.f;
     a.t.half = per ? 0.5f * (float)ix->box : 0.f;
@@ -601,39 +774,41 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     a.st_slot = nl->status;
     a.nl_idx = nl->idx;
     a.nl_d2 = nl->has_d2 ? nl->d2 : nullptr;
+    a.stats = ctx->prof >= 2 ? ctx->stats : nullptr;
     const bool p64 = ix->pos_f64 != 0;
     const dim3 g((unsigned)((nq + KNN_BLOCK - 1) / KNN_BLOCK));
     hipStream_t s = ctx->stream;
     const int need = k + 1;
-    ProfRegion pr1(ctx, "knn_pass1");
+    ProfRegion pr1(ctx, "knn_main");
     if (need <= 5)
-        launch_pass1<5>(a, per, p64, g, s);
+        launch_main<5>(a, per, p64, g, s);
     else if (need <= 9)
-        launch_pass1<9>(a, per, p64, g, s);
+        launch_main<9>(a, per, p64, g, s);
     else if (need <= 17)
-        launch_pass1<17>(a, per, p64, g, s);
+        launch_main<17>(a, per, p64, g, s);
     else if (need <= 21)
-        launch_pass1<21>(a, per, p64, g, s);
+        launch_main<21>(a, per, p64, g, s);
     else if (need <= 33)
-        launch_pass1<33>(a, per, p64, g, s);
+        launch_main<33>(a, per, p64, g, s);
     else if (need <= 65)
-        launch_pass1<65>(a, per, p64, g, s);
+        launch_main<65>(a, per, p64, g, s);
     else if (need <= 101)
-        launch_pass1<101>(a, per, p64, g, s);
+        launch_main<101>(a, per, p64, g, s);
     else
-        launch_pass1<129>(a, per, p64, g, s);
+        launch_main<129>(a, per, p64, g, s);
     pr1.end();
     ProfRegion pr2(ctx, "knn_pass2");
+    const dim3 g2((unsigned)((nq + 255) / 256));
     if (per) {
         if (p64)
-            hipLaunchKernelGGL((k_knn_pass2<true, true>), g, dim3(KNN_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((k_knn_pass2<true, true>), g2, dim3(256), 0, s, a);
         else
-            hipLaunchKernelGGL((k_knn_pass2<true, false>), g, dim3(KNN_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((k_knn_pass2<true, false>), g2, dim3(256), 0, s, a);
     } else {
         if (p64)
-            hipLaunchKernelGGL((k_knn_pass2<false, true>), g, dim3(KNN_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((k_knn_pass2<false, true>), g2, dim3(256), 0, s, a);
         else
-            hipLaunchKernelGGL((k_knn_pass2<false, false>), g, dim3(KNN_BLOCK), 0, s, a);
+            hipLaunchKernelGGL((k_knn_pass2<false, false>), g2, dim3(256), 0, s, a);
     }
     pr2.end();
     ProfRegion pr3(ctx, "knn_resolve");

@@ -38,10 +38,11 @@ def context(device=None) -> ctypes.c_void_p:
     return ctx
 
 
-def prof_enable(on=True, device=None):
-    """Record HIP events around every library launch region."""
+def prof_enable(level=1, device=None):
+    """1: HIP events around every library launch region; 2: also kNN work
+    counters; 0: off."""
     ctx = context(device)
-    raise_for(lib.mhip_prof_enable(ctx, int(bool(on))), ctx)
+    raise_for(lib.mhip_prof_enable(ctx, int(level)), ctx)
 
 
 def prof_read(device=None) -> dict:
