@@ -36,6 +36,7 @@ extern "C" {
 #define MHIP_ST_COUNT 16         /* neighbour collection did not return k members     */
 #define MHIP_ST_SEED_MISS 32     /* learned seed rejected, bracket used               */
 #define MHIP_ST_RANKDEF 64       /* moment matrix rank-deficient by rounding: row NaN */
+#define MHIP_ST_RADIUS_MISS 128  /* search-radius seed too small: recomputed without  */
 
 /* mhip_knn_hsml flags */
 #define MHIP_KNN_KEEP_DIST 1 /* keep squared distances (needed by mhip_nlist_export) */
@@ -90,8 +91,13 @@ int mhip_index_info(const mhip_index *ix, int64_t *n, int *dim, double *boxsize,
  *   qidx     int64[nq] particle index of each row (particle_mask); NULL = all
  *            particles in order (nq is then ignored and taken as n)
  *   m_rows   float64[nq] mass of each row; NULL = 1
- *   seed_h   float32[nq] initial smoothing-length guess per row (learned seed);
- *            NULL = none.  A guess outside the bracket is ignored.
+ *   seed_h   float32[nq] initial smoothing-length guess per row for the Newton
+ *            solve (learned seed); NULL = none.  A guess outside the bracket
+ *            is ignored (MHIP_ST_SEED_MISS).
+ *   seed_r   float32[nq] upper bound guess of the (k+1)-th neighbour distance
+ *            per row; NULL = none.  The search starts with that radius; rows
+ *            where it was too small are recomputed without it
+ *            (MHIP_ST_RADIUS_MISS).  Results never depend on the seeds.
  *   h_out, rho_out, vol_out   float32[nq], any may be NULL
  *   status_out int32[nq] MHIP_ST_* bits, may be NULL
  *   nl       neighbour list filled for later mhip_nlist_export / mhip_apply_D
@@ -100,8 +106,8 @@ int mhip_index_info(const mhip_index *ix, int64_t *n, int *dim, double *boxsize,
 int mhip_nlist_create(mhip_nlist **out);
 int mhip_nlist_destroy(mhip_nlist *nl);
 int mhip_knn_hsml(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
-                  const double *m_rows, const float *seed_h, float *h_out, float *rho_out,
-                  float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl);
+                  const double *m_rows, const float *seed_h, const float *seed_r, float *h_out,
+                  float *rho_out, float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl);
 
 /*
  * Neighbour rows as cKDTree.query returns them (src/meshoid/Meshoid.py:157):
@@ -156,6 +162,32 @@ int mhip_hsml_iter(mhip_ctx *ctx, const void *dists, int dists_f64, int64_t n, i
 
 /* Kernel(q) ufunc (src/meshoid/kernel_density.py:82-96), elementwise, same dtype */
 int mhip_kernel(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out);
+
+/*
+ * Learned smoothing-length seed (no counterpart in the reference, whose
+ * HsmlIter always starts from its bracket, kernel_density.py:37-38).
+ * Features per particle: log(1 + count) of the 3x3x3 coarse cells (1/128 of
+ * the index frame) around it + log(cell size); network 28-64-64-1 SiLU with
+ * bfloat16 weights and float32 accumulation, predicting log(h / cell).
+ *   mhip_seed_features   F float32[n*28], raw features in original row order
+ *   mhip_seed_predict    h_rows float32[n] predicted smoothing length per
+ *                        original row (BF16 MFMA inference); blob from
+ *                        mhip_mlp_pack, mhip_mlp_blob_bytes() bytes
+ *   mhip_mlp_train_step  gradient of the mean squared error over nb samples
+ *                        (F float32[nb*28], T float32[nb] targets log(h/cell))
+ *                        accumulated into grad float32[6081] (zero on entry)
+ *                        and loss float32[1]; step > 0 then applies Adam step
+ *                        `step` with learning rate lr to P float32[6081]
+ *                        (moments m, v) and clears grad.  Parameter order:
+ *                        W1 (64x28), b1, W2 (64x64), b2, w3 (64), b3.
+ */
+int mhip_seed_features(mhip_ctx *ctx, const mhip_index *ix, float *F);
+int mhip_seed_predict(mhip_ctx *ctx, const mhip_index *ix, const void *blob, float *h_rows);
+int mhip_mlp_train_step(mhip_ctx *ctx, const float *F, const float *T, int64_t nb, const float *mu,
+                        const float *sd, float *P, float *m, float *v, float *grad, float *loss,
+                        int step, float lr);
+int mhip_mlp_pack(mhip_ctx *ctx, const float *P, const float *mu, const float *sd, void *blob);
+int64_t mhip_mlp_blob_bytes(void);
 
 #ifdef __cplusplus
 }

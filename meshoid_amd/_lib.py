@@ -17,6 +17,7 @@ ERR_VALUE, ERR_HIP, ERR_SINGULAR, ERR_UNSUPPORTED, ERR_KEY = -1, -2, -3, -4, -5
 ST_AMBIGUOUS, ST_BAND_OVERFLOW, ST_HSML_NOCONV, ST_SINGULAR, ST_COUNT, ST_SEED_MISS = (
     1, 2, 4, 8, 16, 32)
 ST_RANKDEF = 64
+ST_RADIUS_MISS = 128
 KNN_KEEP_DIST = 1
 
 EXPORTED = [
@@ -24,7 +25,8 @@ EXPORTED = [
     "mhip_ctx_synchronize", "mhip_last_error", "mhip_index_create", "mhip_index_build",
     "mhip_index_destroy", "mhip_index_info", "mhip_nlist_create", "mhip_nlist_destroy",
     "mhip_knn_hsml", "mhip_nlist_export", "mhip_apply_D", "mhip_gradient_weights",
-    "mhip_hsml_iter", "mhip_kernel", "mhip_prof_enable", "mhip_prof_read",
+    "mhip_hsml_iter", "mhip_kernel", "mhip_prof_enable", "mhip_prof_read", "mhip_seed_features",
+    "mhip_seed_predict", "mhip_mlp_train_step", "mhip_mlp_pack", "mhip_mlp_blob_bytes",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -53,7 +55,7 @@ _SIGS = {
     "mhip_index_info": ([_p, _p, _p, _p, _p], _i),
     "mhip_nlist_create": ([_pp], _i),
     "mhip_nlist_destroy": ([_p], _i),
-    "mhip_knn_hsml": ([_p, _p, _p, _i64, _i, _p, _p, _p, _p, _p, _p, _i, _p], _i),
+    "mhip_knn_hsml": ([_p, _p, _p, _i64, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p], _i),
     "mhip_nlist_export": ([_p, _p, _p, _p], _i),
     "mhip_apply_D": ([_p, _p, _p, _i, _i64, _i, _i, _i, _p, _p, _p], _i),
     "mhip_gradient_weights": ([_p, _p, _i, _i64, _i, _p, _i64, _i, _p, _p, _d, _i, _i, _p, _p],
@@ -62,6 +64,11 @@ _SIGS = {
     "mhip_kernel": ([_p, _p, _i, _i64, _p], _i),
     "mhip_prof_enable": ([_p, _i], _i),
     "mhip_prof_read": ([_p, ctypes.c_char_p, _i64], _i),
+    "mhip_seed_features": ([_p, _p, _p], _i),
+    "mhip_seed_predict": ([_p, _p, _p, _p], _i),
+    "mhip_mlp_train_step": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _i, ctypes.c_float], _i),
+    "mhip_mlp_pack": ([_p, _p, _p, _p, _p], _i),
+    "mhip_mlp_blob_bytes": ([], _i64),
 }
 for _name, (_args, _res) in _SIGS.items():
     _fn = getattr(lib, _name)

@@ -209,14 +209,48 @@ int mhip_nlist_destroy(mhip_nlist *nl) {
 }
 
 int mhip_knn_hsml(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
-                  const double *m_rows, const float *seed_h, float *h_out, float *rho_out, float *vol_out,
-                  int32_t *status_out, int flags, mhip_nlist *nl) {
+                  const double *m_rows, const float *seed_h, const float *seed_r, float *h_out,
+                  float *rho_out, float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl) {
     if (!ctx) return MHIP_ERR_VALUE;
     MHIP_NEED(ix && nl, "null index or neighbour list");
     MHIP_NEED(ix->n > 0, "index has not been built");
-    hipSetDevice(ctx->device);
-    return knn_run(ctx, ix, qidx, nq, k, m_rows, seed_h, h_out, rho_out, vol_out, status_out, flags, nl);
+    (void)hipSetDevice(ctx->device);
+    return knn_run(ctx, ix, qidx, nq, k, m_rows, seed_h, seed_r, h_out, rho_out, vol_out, status_out,
+                   flags, nl);
 }
+
+int mhip_seed_features(mhip_ctx *ctx, const mhip_index *ix, float *F) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(ix && F, "null index or feature buffer");
+    (void)hipSetDevice(ctx->device);
+    return seed_features(ctx, ix, F);
+}
+
+int mhip_seed_predict(mhip_ctx *ctx, const mhip_index *ix, const void *blob, float *h_rows) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(ix && blob && h_rows, "null argument");
+    (void)hipSetDevice(ctx->device);
+    return seed_predict(ctx, ix, blob, h_rows);
+}
+
+int mhip_mlp_train_step(mhip_ctx *ctx, const float *F, const float *T, int64_t nb, const float *mu,
+                        const float *sd, float *P, float *m, float *v, float *grad, float *loss, int step,
+                        float lr) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(F && T && mu && sd && P && grad && loss, "null argument");
+    MHIP_NEED(step == 0 || (m && v), "Adam state needed for step > 0");
+    (void)hipSetDevice(ctx->device);
+    return mlp_train_step(ctx, F, T, nb, mu, sd, P, m, v, grad, loss, step, lr);
+}
+
+int mhip_mlp_pack(mhip_ctx *ctx, const float *P, const float *mu, const float *sd, void *blob) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(P && mu && sd && blob, "null argument");
+    (void)hipSetDevice(ctx->device);
+    return mlp_pack(ctx, P, mu, sd, blob);
+}
+
+int64_t mhip_mlp_blob_bytes(void) { return (int64_t)mlp_blob_bytes(); }
 
 int mhip_nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out) {
     if (!ctx) return MHIP_ERR_VALUE;

@@ -33,6 +33,7 @@ enum : int32_t {
     ST_COUNT = 16,         // neighbour collection did not return exactly k members
     ST_SEED_MISS = 32,     // learned seed rejected, bracket used instead
     ST_RANKDEF = 64,       // moment matrix rank-deficient at rounding level: row is NaN
+    ST_RADIUS_MISS = 128,  // search-radius seed too small: row recomputed without it
 };
 
 // ---------------------------------------------------------------------------
@@ -186,6 +187,7 @@ struct mhip_index {
     float *nodes = nullptr;
     int64_t node_cap = 0;
     float bbox[6] = {};
+    float frame[4] = {};  // Morton quantisation: origin xyz, scale (cells per unit, 2^21 - 1 over the extent)
 };
 
 struct mhip_nlist {

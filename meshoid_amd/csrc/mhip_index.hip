@@ -445,16 +445,18 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     pr_g.end();
     if ((rc = check_hip(ctx, hipGetLastError(), "index build launch"))) return rc;
     // bbox + nan flag back to the host: periodic range and NaN checks
-    float hb[8];
-    if ((rc = check_hip(ctx, hipMemcpyAsync(hb, bb, 7 * sizeof(float), hipMemcpyDeviceToHost,
+    float hb[12];
+    if ((rc = check_hip(ctx, hipMemcpyAsync(hb, bb, 12 * sizeof(float), hipMemcpyDeviceToHost,
                                             ctx->stream), "bbox readback")))
         return rc;
     if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "index build"))) return rc;
     for (int c = 0; c < 6; c++) ix->bbox[c] = hb[c];
+    for (int c = 0; c < 4; c++) ix->frame[c] = hb[8 + c];
     if (hb[6] != 0.f) return fail(ctx, -1, "positions contain NaN");
     if (box > 0) {
         for (int c = 0; c < dim; c++)
-            if (hb[c] < 0.f || hb[3 + c] >= (float)box)
+            // float64 data just below box can round up to box in float32
+            if (hb[c] < 0.f || hb[3 + c] > (float)box || (hb[3 + c] == (float)box && !pos_f64))
                 return fail(ctx, -1,
                             "Some input data are greater than the size of the periodic box.");
     }

@@ -23,8 +23,17 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
 
 // query setup + pass1/pass2/resolve
 int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
-            const double *m_rows, const float *seed_h, float *h_out, float *rho_out,
-            float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl);
+            const double *m_rows, const float *seed_h, const float *seed_r, float *h_out,
+            float *rho_out, float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl);
+
+// learned smoothing-length seed (mhip_seed.hip)
+int seed_features(mhip_ctx *ctx, const mhip_index *ix, float *F);
+int seed_predict(mhip_ctx *ctx, const mhip_index *ix, const void *blob, float *h_rows);
+int mlp_train_step(mhip_ctx *ctx, const float *F, const float *T, int64_t nb, const float *mu,
+                   const float *sd, float *P, float *m, float *v, float *grad, float *loss, int step,
+                   float lr);
+int mlp_pack(mhip_ctx *ctx, const float *P, const float *mu, const float *sd, void *blob);
+size_t mlp_blob_bytes();
 
 int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out);
 

@@ -185,7 +185,9 @@ struct KnnArgs {
     int64_t nq, n;
     int k, dim;
     const double *m_rows;  // per output row, null = 1
-    const float *seed_h;   // per output row, null = none
+    const float *seed_h;   // Newton seed per output row, null = none
+    const float *seed_r;   // search-radius bound per output row, null = none
+    int redo;              // 1: second launch, redo waves with a radius-seed miss
     float *h_out, *rho_out, *vol_out;
     int32_t *status_out;
     // per slot state
@@ -346,7 +348,15 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     if (wave0 >= a.nq) return;
     const bool live = slot < a.nq;
     const int64_t qs_slot = min<int64_t>(slot, a.nq - 1);
+    // redo launch: only waves where some lane missed its radius seed
+    const bool missed = a.redo && live && (a.st_slot[qs_slot] & ST_RADIUS_MISS);
+    if (a.redo && !__any(missed)) return;
     const uint32_t qp = a.qsorted ? a.qsorted[qs_slot] : (uint32_t)qs_slot;
+    const uint32_t row = a.qrow ? a.qrow[qs_slot] : a.perm[qs_slot];
+    // a radius seed starts the k+1 slots at (seed radius)^2 instead of +inf: a
+    // lane whose (k+1)-th slot still holds that bound at the end missed
+    const float sr = (a.seed_r && !a.redo) ? a.seed_r[row] : 0.f;
+    const float tau0 = sr > 0.f && sr < 3e18f ? sr * sr : INFINITY;
     const TreeView &t = a.t;
     const float *__restrict__ sx = t.sx;
     const float *__restrict__ sy = t.sy;
@@ -366,7 +376,7 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     float A[KCAP];
     const int e = KCAP - 1 - a.k;
 #pragma unroll
-    for (int j = 0; j < KCAP; j++) A[j] = j < e ? -1.0f : INFINITY;
+    for (int j = 0; j < KCAP; j++) A[j] = j < e ? -1.0f : tau0;
     float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
     uint32_t n_ins = 0, n_leaf = 0, n_nodes = 0;
 
@@ -462,13 +472,17 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     }
 
     // (3) epilogue
-    int32_t st = 0;
+    int32_t st = missed ? ST_RADIUS_MISS : 0;
     const float tau = A[KCAP - 2], tau1 = A[KCAP - 1];
+    if (tau0 < INFINITY && tau1 >= tau0) {
+        // radius seed missed: the redo launch recomputes this wave plainly
+        if (live) a.st_slot[slot] = ST_RADIUS_MISS;
+        return;
+    }
     if (tau1 <= tau * (1.0f + AMBIG_REL)) st |= ST_AMBIGUOUS;
     int nlt = 0;
 #pragma unroll
     for (int j = 0; j < KCAP - 1; j++) nlt += (A[j] >= 0.f && A[j] < tau);
-    const uint32_t row = a.qrow ? a.qrow[qs_slot] : a.perm[qs_slot];
     const float seed = a.seed_h ? a.seed_h[row] : 0.f;
     const float h = hsml_solve<KCAP>(A, a.k, a.dim, seed, st);
     if (live) {
@@ -695,8 +709,8 @@ static void launch_main(const KnnArgs &a, bool per, bool p64, dim3 g, hipStream_
 }
 
 int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
-            const double *m_rows, const float *seed_h, float *h_out, float *rho_out, float *vol_out,
-            int32_t *status_out, int flags, mhip_nlist *nl) {
+            const double *m_rows, const float *seed_h, const float *seed_r, float *h_out,
+            float *rho_out, float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl) {
     if (k < 2) return fail(ctx, -1, "des_ngb must be at least 2");
     if (k > 128) return fail(ctx, -4, "des_ngb above 128 is not supported by the GPU kNN");
     if (k > ix->n) return fail(ctx, -1, "des_ngb exceeds the number of particles");
@@ -764,6 +778,8 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     a.dim = ix->dim;
     a.m_rows = m_rows;
     a.seed_h = seed_h;
+    a.seed_r = seed_r;
+    a.redo = 0;
     a.h_out = h_out;
     a.rho_out = rho_out;
     a.vol_out = vol_out;
@@ -796,6 +812,27 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
         launch_main<101>(a, per, p64, g, s);
     else
         launch_main<129>(a, per, p64, g, s);
+    if (seed_r) {
+        // second launch: waves with a radius-seed miss are recomputed plainly
+        KnnArgs r = a;
+        r.redo = 1;
+        if (need <= 5)
+            launch_main<5>(r, per, p64, g, s);
+        else if (need <= 9)
+            launch_main<9>(r, per, p64, g, s);
+        else if (need <= 17)
+            launch_main<17>(r, per, p64, g, s);
+        else if (need <= 21)
+            launch_main<21>(r, per, p64, g, s);
+        else if (need <= 33)
+            launch_main<33>(r, per, p64, g, s);
+        else if (need <= 65)
+            launch_main<65>(r, per, p64, g, s);
+        else if (need <= 101)
+            launch_main<101>(r, per, p64, g, s);
+        else
+            launch_main<129>(r, per, p64, g, s);
+    }
     pr1.end();
     ProfRegion pr2(ctx, "knn_pass2");
     const dim3 g2((unsigned)((nq + 255) / 256));

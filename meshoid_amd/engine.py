@@ -129,26 +129,30 @@ class NeighborList:
 
 
 def knn_hsml(index: SpatialIndex, k: int, qidx=None, m_rows=None, seed_h=None, keep_dist=True,
-             nlist: NeighborList | None = None, out=None):
-    """kNN + smoothing length + density.  Returns dict(h, rho, vol, status, nlist)."""
+             nlist: NeighborList | None = None, out=None, seed_r=None):
+    """kNN + smoothing length + density.  Returns dict(h, rho, vol, status, nlist).
+    seed_h: Newton seed per row; seed_r: search-radius bound per row (rows where
+    it is too small are recomputed without it).  Neither changes the result."""
     dev = index.pos.device
     _need(qidx, (torch.int64,), "qidx")
     _need(m_rows, (torch.float64,), "m_rows")
     _need(seed_h, (torch.float32,), "seed_h")
+    _need(seed_r, (torch.float32,), "seed_r")
     nq = index.n if qidx is None else qidx.numel()
     if qidx is not None and nq and (int(qidx.min()) < 0 or int(qidx.max()) >= index.n):
         raise IndexError("query index out of range")
     if m_rows is not None and m_rows.numel() != nq:
         raise ValueError("m_rows must have one entry per query row")
-    if seed_h is not None and seed_h.numel() != nq:
-        raise ValueError("seed_h must have one entry per query row")
+    for sname, sv in (("seed_h", seed_h), ("seed_r", seed_r)):
+        if sv is not None and sv.numel() != nq:
+            raise ValueError(f"{sname} must have one entry per query row")
     if out is None:
         out = {name: torch.empty(nq, dtype=torch.float32, device=dev) for name in ("h", "rho", "vol")}
         out["status"] = torch.empty(nq, dtype=torch.int32, device=dev)
     nl = nlist if nlist is not None else NeighborList()
     ctx = context(dev)
     rc = lib.mhip_knn_hsml(ctx, index.h, ptr(qidx), nq, int(k), ptr(m_rows), ptr(seed_h),
-                           ptr(out.get("h")), ptr(out.get("rho")), ptr(out.get("vol")),
+                           ptr(seed_r), ptr(out.get("h")), ptr(out.get("rho")), ptr(out.get("vol")),
                            ptr(out.get("status")), _lib.KNN_KEEP_DIST if keep_dist else 0, nl.h)
     raise_for(rc, ctx)
     nl.nq, nl.k, nl.index, nl.keep_dist = nq, int(k), index, keep_dist
