@@ -112,6 +112,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stats", action="store_true", help="kNN work counters (slower)")
     ap.add_argument("--plain-only", action="store_true", help="skip the learned-seed form")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a known-size kernel "
+                         "that scales the profiler's HBM byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -128,6 +131,14 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     n, k, K = args.n, args.k, 1
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
     log(f"generating {n} clustered particles")
     pos = synthetic.clustered(n, seed=args.seed, nclump=256, background=0.1)
     f, _ = synthetic.sin3(pos)

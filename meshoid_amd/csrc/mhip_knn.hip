@@ -36,7 +36,6 @@ namespace mhip {
 constexpr int KNN_BLOCK = 256;
 constexpr int KNN_WAVES = KNN_BLOCK / WAVE;
 constexpr int QCAP = 32;                     // per-lane pending-candidate queue
-constexpr int STACK = FANOUT * (MAX_DEPTH + 1);  // per-wave traversal stack
 constexpr int WLCAP = 384;                   // per-wave record of candidate leaves
 constexpr float U32 = 5.9604645e-8f;         // 2^-24
 constexpr float AMBIG_REL = 12.0f * U32;     // fp32 vs fp64 ordering ambiguity of d^2
@@ -52,6 +51,7 @@ struct TreeView {
     float box, half;  // periodic box length and half of it (0: open)
     float slack;      // absolute box-test slack: 0 for float32 inputs, covers
                       // the float32 residuals of float64 inputs otherwise
+    int depth;        // level of the leaves (root = level 0)
 };
 
 struct QPos {
@@ -158,17 +158,23 @@ MHIP_DEV void load_leaf(const TreeView &t, int64_t leaf, float (&cx)[LEAF], floa
     }
 }
 
+// Inserts x into the ascending list A, dropping the largest entry: slot j
+// becomes med3(A[j-1], x, A[j]).  The slots are updated in place from the top
+// down (each instruction's destination is its own old value), so the list
+// keeps its registers across loop iterations instead of alternating between
+// two register sets joined by copies.  x is never NaN.
 template <int KCAP>
 MHIP_DEV void insert_sorted(float (&A)[KCAP], float x) {
 #pragma unroll
-    for (int j = KCAP - 1; j > 0; --j) A[j] = med3(A[j - 1], x, A[j]);
-    A[0] = fminf(A[0], x);
+    for (int j = KCAP - 1; j > 0; --j)
+        asm volatile("v_med3_f32 %0, %1, %2, %0" : "+v"(A[j]) : "v"(A[j - 1]), "v"(x));
+    asm volatile("v_min_f32 %0, %1, %0" : "+v"(A[0]) : "v"(x));
 }
 
 MHIP_DEV int wave_max(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
+    return (int)uniform((uint32_t)v);
 }
 
 MHIP_DEV uint32_t wave_sum(uint32_t v) {
@@ -337,10 +343,82 @@ MHIP_DEV float node_d2_u(const TreeView &t, const QPos &q, uint32_t x) {
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
+// squared gap from a lane's query to a box read at wave-uniform addresses
+// (scalar loads); the query interval [ql, qh] per axis is the position widened
+// by the float64-residual slack (ql = qh for float32 inputs); zero inside
+template <bool PER>
+MHIP_DEV float axis_gap_u(float ql, float qh, float lo, float hi, float box) {
+    float g = fmaxf(lo - qh, ql - hi);
+    if (PER) {
+        g = fminf(g, fmaxf((lo - box) - qh, ql - (hi - box)));  // the box one period lower
+        g = fminf(g, fmaxf(lo - (qh - box), (ql - box) - hi));  // the box one period higher
+    }
+    return fmaxf(g, 0.f);
+}
+
+struct QBox {
+    float lx, ly, lz, hx, hy, hz;
+};
+
+template <bool PER>
+MHIP_DEV float box_gap2_u(const TreeView &t, const QBox &b, uint32_t x) {
+    const int64_t o = (int64_t)x * 8;
+    const float gx = axis_gap_u<PER>(b.lx, b.hx, ldc(t.nodes, o), ldc(t.nodes, o + 3), t.box);
+    const float gy = axis_gap_u<PER>(b.ly, b.hy, ldc(t.nodes, o + 1), ldc(t.nodes, o + 4), t.box);
+    const float gz = axis_gap_u<PER>(b.lz, b.hz, ldc(t.nodes, o + 2), ldc(t.nodes, o + 5), t.box);
+    return fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+}
+
+// bit c set: child 8x+1+c lies within bound of some lane's query
+template <bool PER>
+MHIP_DEV uint32_t expand_mask(const TreeView &t, const QBox &b, uint32_t x, float bound) {
+    uint32_t m = 0;
+    const uint32_t c0 = FANOUT * x + 1;
+#pragma unroll
+    for (int c = 0; c < FANOUT; c++)
+        if (__any(box_gap2_u<PER>(t, b, c0 + c) <= bound)) m |= 1u << c;
+    return m;
+}
+
+// the set bit of m nearest to position o (m != 0); ties go upwards
+MHIP_DEV uint32_t pick_nearest(uint32_t m, uint32_t o) {
+    const uint32_t up = m >> o;
+    const uint32_t down = m & ((1u << o) - 1u);
+    if (!down) return o + __builtin_ctz(up);
+    const uint32_t cd = 31u - __builtin_clz(down);
+    if (!up) return cd;
+    const uint32_t cu = o + __builtin_ctz(up);
+    return cu - o <= o - cd ? cu : cd;
+}
+
+// 8-bit child masks of the traversal path, levels 0..7 and 8..15
+struct MaskStack {
+    uint64_t lo = 0, hi = 0;
+    MHIP_DEV uint32_t get(int l) const {
+        return l < 8 ? (uint32_t)(lo >> (8 * l)) & 0xffu : (uint32_t)(hi >> (8 * (l - 8))) & 0xffu;
+    }
+    MHIP_DEV void put(int l, uint32_t m) {
+        if (l < 8)
+            lo |= (uint64_t)m << (8 * l);
+        else
+            hi |= (uint64_t)m << (8 * (l - 8));
+    }
+    MHIP_DEV void clear(int l, uint32_t c) {
+        if (l < 8)
+            lo &= ~(1ull << (8 * l + c));
+        else
+            hi &= ~(1ull << (8 * (l - 8) + c));
+    }
+};
+
+constexpr int LROWS = 33;                          // staged member rows (k <= 32)
+constexpr int QROWS = QCAP > LROWS ? QCAP : LROWS;  // per-lane LDS rows
+
 template <int KCAP, bool PER, bool P64>
 __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
-    __shared__ float s_queue[KNN_WAVES][QCAP][WAVE];
-    __shared__ uint32_t s_stack[KNN_WAVES][STACK];
+    // members are staged in LDS and written as whole rows when they fit
+    constexpr bool STAGE = KCAP <= LROWS;
+    __shared__ uint32_t s_buf[KNN_WAVES][QROWS][WAVE];
     __shared__ uint32_t s_wl[KNN_WAVES][WLCAP];
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
     int64_t wave0;
@@ -365,11 +443,13 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     const float *__restrict__ ly = t.ly;
     const float *__restrict__ lz = t.lz;
     const QPos q = load_q<P64>(t, qp);
-    QPos centre;  // the wave's middle query orders the children near-to-far
-    centre.x = __shfl(q.x, 31);
-    centre.y = __shfl(q.y, 31);
-    centre.z = __shfl(q.z, 31);
-    centre.lx = centre.ly = centre.lz = 0.f;
+    QBox qb;
+    qb.lx = P64 ? q.x - t.slack : q.x;
+    qb.ly = P64 ? q.y - t.slack : q.y;
+    qb.lz = P64 ? q.z - t.slack : q.z;
+    qb.hx = P64 ? q.x + t.slack : q.x;
+    qb.hy = P64 ? q.y + t.slack : q.y;
+    qb.hz = P64 ? q.z + t.slack : q.z;
 
     // sorted k+1 smallest squared distances; slots below e are -1 sentinels
     // when k + 1 < KCAP, so A[KCAP-2] is the k-th and A[KCAP-1] the (k+1)-th
@@ -380,88 +460,108 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
     uint32_t n_ins = 0, n_leaf = 0, n_nodes = 0;
 
-    // (1) priming: the leaves holding this wave's own queries, inserted directly
+    // (1) the leaves holding this wave's own queries first (they fill every
+    // list with a nearby bound), then (2) a wave-uniform depth-first traversal
+    // without a stack: expanding a node tests all 8 children against every
+    // lane's bound at once and keeps the survivors as an 8-bit mask per level;
+    // children nearest (in index) to the path of the wave's middle query are
+    // taken first.  Every candidate of a leaf is written to its lane's LDS queue
+    // row qn, and qn advances when the candidate is under the lane's current
+    // (k+1)-th distance; the queues drain into the sorted lists in batches.
+    // Leaves where some lane queued a candidate are recorded for (3).
     const uint32_t p_first = uniform(__shfl(qp, 0));
     const uint32_t p_last = uniform(__shfl(qp, WAVE - 1));
     const int64_t l0 = p_first / LEAF;
     const int64_t l1 = min<int64_t>(p_last / LEAF, l0 + 7);
-    for (int64_t L = l0; L <= l1; L++) {
-        load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, L, cx, cy, cz, clx, cly, clz);
-        n_leaf++;
-#pragma unroll
-        for (int i = 0; i < LEAF; i++) {
-            const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
-            const bool ins = d2 < A[KCAP - 1];
-            if (__any(ins)) {
-                insert_sorted<KCAP>(A, ins ? d2 : INFINITY);
-                n_ins++;
-            }
-        }
-    }
-
-    // (2) wave-uniform depth-first traversal, children near-to-far through a
-    // per-wave LDS stack.  Candidates under a lane's current (k+1)-th distance
-    // go to its LDS queue; the queues drain into the sorted lists in batches.
-    // Leaves where some lane queued a candidate are recorded for (3).
+    float t1 = A[KCAP - 1];
+    float bound = t1 * PRUNE_REL;
     int qn = 0;
-    auto drain = [&]() {
-        const int m = wave_max(qn);
-        for (int i = 0; i < m; i++) {
-            const float v = i < qn ? s_queue[w][i][lane] : INFINITY;
-            insert_sorted<KCAP>(A, v);
-        }
-        n_ins += m;
-        qn = 0;
-    };
-    uint32_t *stk = s_stack[w];
     uint32_t nwl = 0;  // recorded leaves (uniform)
     const uint32_t first_leaf = (uint32_t)t.first_leaf;
-    if (lane == 0) stk[0] = 0u;
-    int sp = 1;
-    __builtin_amdgcn_wave_barrier();
-    while (sp > 0) {
-        sp--;
-        const uint32_t x = uniform(stk[sp]);
-        n_nodes++;
-        if (!__any(node_d2_u<PER>(t, q, x) <= A[KCAP - 1] * PRUNE_REL)) continue;
-        if (x >= first_leaf) {
-            const int64_t li = x - first_leaf;
-            // skip primed and empty leaves (empty ones have +inf boxes, which
-            // pass while some list is not full)
-            if ((li >= l0 && li <= l1) || li >= t.nleaf) continue;
+    const int depth = t.depth;
+    const uint32_t lc = uniform(__shfl(qp, WAVE / 2 - 1)) / LEAF;
+    MaskStack ms;
+    int lvl = 0;
+    uint32_t x = 0, off = 0;  // node whose children (level lvl + 1) are pending; its offset in its level
+    int64_t L = l0;           // next own leaf
+    bool walking = false, done = false;
+    // batches: scan leaves until some queue is nearly full (the sorted lists
+    // are not touched there), then drain every queue into its list
+    while (true) {
+        while (true) {
+            int64_t li = -1;
+            if (L <= l1) {
+                li = L++;
+            } else if (depth > 0) {
+                if (!walking) {
+                    walking = true;
+                    ms.put(0, expand_mask<PER>(t, qb, 0u, bound));
+                    n_nodes += FANOUT;
+                }
+                while (true) {
+                    const uint32_t m = ms.get(lvl);
+                    if (m == 0u) {
+                        if (lvl == 0) break;
+                        lvl--;
+                        x = (x - 1) >> 3;
+                        off >>= 3;
+                        continue;
+                    }
+                    // the child on (or nearest to) the middle query's path first
+                    const uint32_t anc = lc >> (3 * (depth - 1 - lvl));
+                    const uint32_t base = FANOUT * off;
+                    const uint32_t o = anc < base ? 0u : (anc > base + 7 ? 7u : anc - base);
+                    const uint32_t c = pick_nearest(m, o);
+                    ms.clear(lvl, c);
+                    const uint32_t y = FANOUT * x + 1 + c;
+                    if (lvl + 1 < depth) {
+                        lvl++;
+                        x = y;
+                        off = base + c;
+                        ms.put(lvl, expand_mask<PER>(t, qb, y, bound));
+                        n_nodes += FANOUT;
+                        continue;
+                    }
+                    const int64_t cand = (int64_t)(y - first_leaf);
+                    // own leaves are done; empty leaves pass the box test
+                    // only while some list is not full
+                    if ((cand >= l0 && cand <= l1) || cand >= t.nleaf) continue;
+                    li = cand;
+                    break;
+                }
+            }
+            if (li < 0) {
+                done = true;
+                break;
+            }
             n_leaf++;
             load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, li, cx, cy, cz, clx, cly, clz);
             const int before = qn;
 #pragma unroll
             for (int i = 0; i < LEAF; i++) {
                 const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
-                if (d2 < A[KCAP - 1]) {
-                    s_queue[w][qn][lane] = d2;
-                    qn++;
-                }
+                // qn <= QCAP - LEAF when a leaf starts, so the row stays in range
+                s_buf[w][qn][lane] = __float_as_uint(d2);
+                qn += d2 < t1;
             }
             if (__any(qn > before)) {
                 if (nwl < WLCAP && lane == 0) s_wl[w][nwl] = (uint32_t)li;
                 nwl++;
             }
-            if (__any(qn > QCAP - LEAF)) drain();
-        } else {
-            // children 8x+1..8x+8, pushed far-to-near so the nearest pops first
-            const uint32_t c0 = FANOUT * x + 1;
-            float cd = INFINITY;
-            if (lane < FANOUT) cd = node_d2<PER>(t, centre, c0 + lane);
-            int rank = 0;
-#pragma unroll
-            for (int o = 0; o < FANOUT; o++) {
-                const float od = __shfl(cd, o);
-                rank += od < cd || (od == cd && o < lane);
-            }
-            if (lane < FANOUT) stk[sp + FANOUT - 1 - rank] = c0 + lane;
-            __builtin_amdgcn_wave_barrier();
-            sp += FANOUT;
+            if (__any(qn > QCAP - LEAF)) break;
         }
+        const int nd = wave_max(qn);
+        for (int i = 0; i < nd; i++) {
+            // rows below QCAP are always in range; rows at or past qn are stale
+            const float r = __uint_as_float(s_buf[w][i][lane]);
+            insert_sorted<KCAP>(A, i < qn ? r : INFINITY);
+        }
+        n_ins += nd;
+        qn = 0;
+        t1 = A[KCAP - 1];
+        bound = t1 * PRUNE_REL;
+        if (done) break;
     }
-    drain();
     __builtin_amdgcn_wave_barrier();
     if (a.stats && lane == 0) {
         atomicAdd(a.stats + STAT_P1_LEAVES, (unsigned long long)n_leaf);
@@ -497,34 +597,46 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         if (a.rho_out) a.rho_out[row] = (float)rho;
         if (a.vol_out) a.vol_out[row] = (float)(m / rho);
     }
-    // members: a candidate at or below tau was inserted while priming or
-    // passed its lane's queue filter when its leaf was scanned, so it lies in
-    // a primed or recorded leaf; leaves no final ball reaches are skipped
+    // members: a candidate at or below tau passed its lane's queue filter
+    // when its leaf was scanned, so it lies in a recorded leaf; leaves no
+    // final ball reaches are skipped
     if (nwl <= WLCAP) {
         int cnt = 0, eq_left = a.k - nlt;
-        const uint32_t total = (uint32_t)(l1 - l0 + 1) + nwl;
-        for (uint32_t r = 0; r < total; r++) {
-            const int64_t li = r <= (uint32_t)(l1 - l0) ? l0 + r : (int64_t)uniform(s_wl[w][r - (l1 - l0 + 1)]);
-            if (!__any(node_d2_u<PER>(t, q, first_leaf + (uint32_t)li) <= tau * PRUNE_REL)) continue;
+        const float tb = tau * PRUNE_REL;
+        for (uint32_t r = 0; r < nwl; r++) {
+            const uint32_t li = uniform(s_wl[w][r]);
+            if (!__any(box_gap2_u<PER>(t, qb, first_leaf + li) <= tb)) continue;
             load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, li, cx, cy, cz, clx, cly, clz);
 #pragma unroll
             for (int i = 0; i < LEAF; i++) {
                 const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
                 bool take = d2 < tau;
-                if (d2 == tau && eq_left > 0) {
-                    take = true;
-                    eq_left--;
+                const bool tie = d2 == tau;
+                if (__any(tie)) {
+                    if (tie && eq_left > 0) {
+                        take = true;
+                        eq_left--;
+                    }
                 }
-                if (take && live && cnt < a.k) {
-                    a.nl_idx[(int64_t)cnt * a.nq + slot] = (uint32_t)(li * LEAF + i);
-                    if (a.nl_d2) a.nl_d2[(int64_t)cnt * a.nq + slot] = d2;
-                    cnt++;
+                take = take && cnt < a.k;
+                if (a.nl_d2 && take && live) a.nl_d2[(int64_t)cnt * a.nq + slot] = d2;
+                if (STAGE) {
+                    // row cnt <= k < LROWS; the row is kept only on a hit
+                    s_buf[w][cnt][lane] = li * LEAF + i;
+                } else if (take && live) {
+                    a.nl_idx[(int64_t)cnt * a.nq + slot] = li * LEAF + i;
                 }
+                cnt += take;
             }
+        }
+        if (STAGE && live) {
+            for (int j = 0; j < a.k; j++) a.nl_idx[(int64_t)j * a.nq + slot] = j < cnt ? s_buf[w][j][lane] : qp;
         }
         if (live && cnt != a.k) {
             st |= ST_COUNT;
-            pad_list(a, cnt, qp, slot);
+            if (!STAGE) pad_list(a, cnt, qp, slot);
+            else if (a.nl_d2)
+                for (int j = cnt; j < a.k; j++) a.nl_d2[(int64_t)j * a.nq + slot] = INFINITY;
         }
     } else {
         st |= ST_REDO;
@@ -761,6 +873,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     a.t.nodes = ix->nodes;
     a.t.first_leaf = ix->first_leaf;
     a.t.nleaf = ix->nleaf;
+    a.t.depth = ix->depth;
     const bool per = ix->box > 0;
     a.t.box = per ? (float)ix->box : 0.f;
     a.t.half = per ? 0.5f * (float)ix->box : 0.f;
