@@ -222,8 +222,8 @@ def main():
     dom = max((r for r in kernels if alg_bytes(r, k, K) > 0), key=lambda r: kernels[r])
     ach = alg_bytes(dom, k, K) * n / (kernels[dom] * 1e-3)
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
-    if os.path.exists(tfile):
+    tfile = os.path.join(ROOT, "profiles", f"{dom}_clustered_2e24_k{k}_traffic.json")
+    if n == 1 << 24 and args.seed == 0 and os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 

@@ -129,7 +129,7 @@ enum : int {
     STAT_P1_INSERTS,      // insertion steps executed per wavefront (sum over waves)
     STAT_P1_NODES,        // tree nodes tested (sum over lanes)
     STAT_P2_LEAVES,       // leaves scanned in pass 2
-    STAT_P2_NODES,        // tree nodes popped in pass 2
+    STAT_P2_NODES,        // recorded leaves re-scanned for members (main pass)
     STAT_WAVES,           // wavefronts of pass 1
     STAT_COUNT
 };

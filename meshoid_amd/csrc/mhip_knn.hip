@@ -247,13 +247,16 @@ MHIP_DEV void pad_list(const KnnArgs &a, int cnt, uint32_t qp, int64_t slot) {
 // ---------------------------------------------------------------------------
 // smoothing length on the k nearest distances (kernel_density.py:36-56 root):
 // norm * sum_j W(r_j/h) = k, bracket [r_1, c_d r_{k-1}].  Safeguarded Newton
-// with compensated float32 sums; the bracket is the fallback.
+// with compensated float32 sums; the bracket is the fallback.  On return
+// A[0..KCAP-2] hold the distances (sentinel slots +inf), not their squares:
+// the list is not needed afterwards, and reusing its registers keeps the
+// kernel's register count at that of the traversal.
 // ---------------------------------------------------------------------------
 template <int KCAP>
-MHIP_DEV float hsml_solve(const float (&A)[KCAP], int k, int dim, float seed, int32_t &st) {
+MHIP_DEV float hsml_solve(float (&A)[KCAP], int k, int dim, float seed, int32_t &st) {
     const int e = KCAP - 1 - k;  // first real slot
     const float norm = (float)hsml_norm(dim);
-    float r[KCAP - 1];
+    float(&r)[KCAP] = A;
     float lo = 0.f, rmax = 0.f;
 #pragma unroll
     for (int j = 0; j < KCAP - 1; j++) {
@@ -283,6 +286,9 @@ MHIP_DEV float hsml_solve(const float (&A)[KCAP], int k, int dim, float seed, in
             c = (tt - s) - y;
             s = tt;
             ds += q <= 1.0f ? dw * q : 0.0f;
+            // keep the terms in order: evaluating all k kernel values ahead
+            // of the sums would hold them all in registers at once
+            asm volatile("" : "+v"(s), "+v"(c), "+v"(ds));
         }
         const float f = norm * s - kf;
         if (f == 0.f) break;
@@ -554,9 +560,13 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         for (int i = 0; i < nd; i++) {
             // rows below QCAP are always in range; rows at or past qn are stale
             const float r = __uint_as_float(s_buf[w][i][lane]);
-            insert_sorted<KCAP>(A, i < qn ? r : INFINITY);
+            const float v = i < qn ? r : INFINITY;
+            // rows that no lane's list takes any more are skipped
+            if (__any(v < A[KCAP - 1])) {
+                insert_sorted<KCAP>(A, v);
+                n_ins++;
+            }
         }
-        n_ins += nd;
         qn = 0;
         t1 = A[KCAP - 1];
         bound = t1 * PRUNE_REL;
@@ -601,24 +611,22 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     // when its leaf was scanned, so it lies in a recorded leaf; leaves no
     // final ball reaches are skipped
     if (nwl <= WLCAP) {
-        int cnt = 0, eq_left = a.k - nlt;
+        // Unless the row is ambiguous, exactly k particles have d2 <= tau
+        // (the (k+1)-th distance is strictly larger), so "d2 <= tau" selects
+        // the members.  Ambiguous rows (float32 ties at tau included) are
+        // rebuilt by the float64 resolver; for them the count is only capped.
+        int cnt = 0;
+        uint32_t nscan = 0;
         const float tb = tau * PRUNE_REL;
         for (uint32_t r = 0; r < nwl; r++) {
             const uint32_t li = uniform(s_wl[w][r]);
             if (!__any(box_gap2_u<PER>(t, qb, first_leaf + li) <= tb)) continue;
+            nscan++;
             load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, li, cx, cy, cz, clx, cly, clz);
 #pragma unroll
             for (int i = 0; i < LEAF; i++) {
                 const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
-                bool take = d2 < tau;
-                const bool tie = d2 == tau;
-                if (__any(tie)) {
-                    if (tie && eq_left > 0) {
-                        take = true;
-                        eq_left--;
-                    }
-                }
-                take = take && cnt < a.k;
+                const bool take = (d2 <= tau) & (cnt < a.k);
                 if (a.nl_d2 && take && live) a.nl_d2[(int64_t)cnt * a.nq + slot] = d2;
                 if (STAGE) {
                     // row cnt <= k < LROWS; the row is kept only on a hit
@@ -629,6 +637,7 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
                 cnt += take;
             }
         }
+        if (a.stats && lane == 0) atomicAdd(a.stats + STAT_P2_NODES, (unsigned long long)nscan);
         if (STAGE && live) {
             for (int j = 0; j < a.k; j++) a.nl_idx[(int64_t)j * a.nq + slot] = j < cnt ? s_buf[w][j][lane] : qp;
         }
