@@ -78,8 +78,24 @@ def cpu_baseline(n, k, seed):
                       f"einsum ({orc.num_threads()} threads), {dt:.2f} s"}
 
 
-def accuracy(pos, f, h, rho, d1, k, nsample, seed):
-    """Errors of the GPU result against the FP64 oracle on a random sample."""
+def _rel(a, b):
+    """Per-row relative error |a - b| / |b| over the trailing axes."""
+    a = a.reshape(len(a), -1)
+    b = b.reshape(len(b), -1)
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+
+
+def _summ(e):
+    ok = np.isfinite(e)
+    e = e[ok]
+    return {"max": float(e.max()), "p99999": float(np.quantile(e, 0.99999)),
+            "median": float(np.median(e)), "excluded_rank_deficient": int((~ok).sum())}
+
+
+def accuracy(pos, f, h, rho, d1, k, nsample, seed, fields2=None, d1o2=None, d2=None):
+    """Errors of the GPU result against the FP64 oracle on a random sample.
+    With fields2 (n, K) and the order-2 outputs d1o2 (n, K, 3), d2 (n, K, 6):
+    also the order-2 derivatives (Meshoid.D2) of each field."""
     from oracle import oracle as orc
 
     rng = np.random.default_rng(seed)
@@ -92,11 +108,42 @@ def accuracy(pos, f, h, rho, d1, k, nsample, seed):
     rho_o = k / (4 * np.pi / 3 * h_o**3)
     eh = np.abs(h[idx] / h_o - 1)
     er = np.abs(rho[idx] / rho_o - 1)
-    eg = np.linalg.norm(d1[idx] - g_o, axis=1) / np.maximum(np.linalg.norm(g_o, axis=1), 1e-300)
-    return {"sample": int(nsample), "h_max_rel": float(eh.max()), "h_median_rel": float(np.median(eh)),
-            "rho_max_rel": float(er.max()), "gradf_max_rel": float(eg.max()),
-            "gradf_p99999_rel": float(np.quantile(eg, 0.99999)),
-            "gradf_median_rel": float(np.median(eg))}
+    eg = _rel(d1[idx], g_o)
+    out = {"sample": int(nsample), "h_max_rel": float(eh.max()), "h_median_rel": float(np.median(eh)),
+           "rho_max_rel": float(er.max()), "gradf_max_rel": float(eg.max()),
+           "gradf_p99999_rel": float(np.quantile(eg, 0.99999)),
+           "gradf_median_rel": float(np.median(eg))}
+    if fields2 is not None:
+        G2 = orc.gradient_weights(p64, ngb, h_o, idx, order=2)
+        o2 = orc.apply_weights(G2, ngb, idx, fields2.astype(np.float64))  # (m, K, 9)
+        # condition number of each row's 9x9 moment matrix E = sum_j w a a^T
+        # (derivatives.py:83-101): a float64 solve loses ~cond(E) * 1e-16, so
+        # the oracle itself is only meaningful where cond(E) is moderate
+        dx = p64[ngb] - p64[idx][:, None, :]
+        w = orc.kernel(np.linalg.norm(dx, axis=2) / h_o[:, None])
+        x, y, z = dx[..., 0], dx[..., 1], dx[..., 2]
+        a = np.stack([x, y, z, x * x / 2, y * y / 2, z * z / 2, x * y, y * z, z * x], axis=2)
+        E = np.einsum("ij,ija,ijb->iab", w, a, a)
+        # after symmetric diagonal equilibration: the first- and second-order
+        # blocks scale as h^2 and h^4, which alone would dominate cond(E)
+        dg = np.sqrt(np.einsum("iaa->ia", E))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            cond = np.linalg.cond(E / (dg[:, :, None] * dg[:, None, :]))
+        cond = np.where(np.isfinite(cond), cond, np.inf)
+        prec = {"cond_E_quantiles_50_99_max": [float(np.median(cond)), float(np.quantile(cond, 0.99)),
+                                               float(cond.max())]}
+        for j, name in enumerate(("sin", "grf")[: fields2.shape[1]]):
+            eg = _rel(d1o2[idx, j], o2[:, j, :3])
+            eh = _rel(d2[idx, j], o2[:, j, 3:])
+            rec = {}
+            for lim in (1e8, 1e12):
+                ok = cond <= lim
+                rec[f"cond<={lim:.0e}"] = {"rows": int(ok.sum()), "grad_order2_rel": _summ(eg[ok]),
+                                           "hessian_rel": _summ(eh[ok])}
+            rec["all_rows"] = {"grad_order2_rel": _summ(eg), "hessian_rel": _summ(eh)}
+            prec[name] = rec
+        out["order2"] = prec
+    return out
 
 
 def main():
@@ -229,11 +276,34 @@ def main():
 
     status = st.cpu().numpy() | out["status"].cpu().numpy()
     acc = None
+    precision = None
     if args.check:
+        # precision form: order-2 derivatives (Meshoid.D2) of f and of a
+        # Gaussian random field (P(k) ~ k^-3, periodic 256^3 grid over the
+        # 100-unit box), on the same neighbour lists
+        log("precision form: order-2 derivatives of sin and GRF fields")
+        grf = synthetic.grf(pos, seed=args.seed + 7)
+        F2 = np.stack([f.astype(np.float64), grf], axis=1).astype(np.float32)
+        F2_d = torch.from_numpy(F2).to(dev)
+        o1 = torch.empty((n, 2, 3), dtype=torch.float32, device=dev)
+        o2 = torch.empty((n, 2, 6), dtype=torch.float32, device=dev)
+        st2 = torch.empty(n, dtype=torch.int32, device=dev)
+
+        def order2():
+            engine.apply_D(nl, F2_d, order=2, want_d2=True, out1=o1, out2=o2, status=st2)
+
+        dt2, kern2, _ = timed(order2, 3, 1, 1)
+        precision = {"fields": ["sin(2 pi x/10) sin(2 pi y/10) sin(2 pi z/10)",
+                                "GRF P(k)~k^-3, 256^3 periodic grid, trilinear"],
+                     "order2_ms": dt2 * 1e3, "order2_kernel_ms": {a: round(b, 4) for a, b in kern2.items()},
+                     "rank_deficient_rows": int(((st2.cpu().numpy() & 64) != 0).sum())}
         log("oracle accuracy sample")
         acc = accuracy(pos, f, out["h"].cpu().numpy().astype(np.float64),
                        out["rho"].cpu().numpy().astype(np.float64),
-                       d1[:, 0, :].cpu().numpy().astype(np.float64), k, args.check, args.seed + 1)
+                       d1[:, 0, :].cpu().numpy().astype(np.float64), k, args.check, args.seed + 1,
+                       fields2=F2, d1o2=o1.cpu().numpy().astype(np.float64),
+                       d2=o2.cpu().numpy().astype(np.float64))
+        precision["errors_vs_fp64_oracle"] = acc.pop("order2")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -264,6 +334,7 @@ def main():
                           "rank_deficient": int(((status & 64) != 0).sum()),
                           "hsml_noconv": int(((status & 4) != 0).sum())},
         "accuracy": acc,
+        "precision": precision,
         "cpu_baseline": cpu,
         "library": meshoid_amd._lib.LIB_PATH,
     }

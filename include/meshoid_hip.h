@@ -164,6 +164,38 @@ int mhip_hsml_iter(mhip_ctx *ctx, const void *dists, int dists_f64, int64_t n, i
 int mhip_kernel(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out);
 
 /*
+ * Meshoid.get_kernel_weights (src/meshoid/Meshoid.py:172-177): W(d/h)
+ * normalised per row, for a row-major (nq, k) float32 distance array (the
+ * exported ngbdist) and float32 h[nq]; w float64[nq*k].  A row with a zero
+ * kernel sum is NaN, as in the reference.
+ */
+int mhip_kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w);
+
+/*
+ * Meshoid.KernelAverage(f) (src/meshoid/Meshoid.py:369-381): per query row,
+ * sum_j W(r_j/h) f[ngb_j] / sum_j W(r_j/h) over the list's neighbours, scalar
+ * f (nf_rows) float32/float64 in original row order; out float64[nq] in row
+ * order.  Pair distances are recomputed in float64.
+ */
+int mhip_kernel_average(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows,
+                        double *out);
+
+/*
+ * Meshoid.KernelVariance(f) (src/meshoid/Meshoid.py:352-367): population
+ * standard deviation np.std(f[ngb], axis=1) of each of the K components of
+ * f (nf_rows, K); out float64[nq*K], rows in row order.
+ */
+int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows,
+                         int K, double *out);
+
+/*
+ * Meshoid.Integrate(f) (src/meshoid/Meshoid.py:333-350): sum_i vol_i f_i per
+ * component, vol float32[n], f (n, K) float32/float64 (rows already masked);
+ * out float64[K].  Deterministic two-level float64 reduction.
+ */
+int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
+
+/*
  * Learned smoothing-length seed (no counterpart in the reference, whose
  * HsmlIter always starts from its bracket, kernel_density.py:37-38).
  * Features per particle: log(1 + count) of the 3x3x3 coarse cells (1/128 of

@@ -27,6 +27,7 @@ EXPORTED = [
     "mhip_knn_hsml", "mhip_nlist_export", "mhip_apply_D", "mhip_gradient_weights",
     "mhip_hsml_iter", "mhip_kernel", "mhip_prof_enable", "mhip_prof_read", "mhip_seed_features",
     "mhip_seed_predict", "mhip_mlp_train_step", "mhip_mlp_pack", "mhip_mlp_blob_bytes",
+    "mhip_kernel_weights", "mhip_kernel_average", "mhip_kernel_variance", "mhip_integrate",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -69,6 +70,10 @@ _SIGS = {
     "mhip_mlp_train_step": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _i, ctypes.c_float], _i),
     "mhip_mlp_pack": ([_p, _p, _p, _p, _p], _i),
     "mhip_mlp_blob_bytes": ([], _i64),
+    "mhip_kernel_weights": ([_p, _p, _p, _i64, _i, _p], _i),
+    "mhip_kernel_average": ([_p, _p, _p, _i, _i64, _p], _i),
+    "mhip_kernel_variance": ([_p, _p, _p, _i, _i64, _i, _p], _i),
+    "mhip_integrate": ([_p, _p, _p, _i, _i64, _i, _p], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     _fn = getattr(lib, _name)

@@ -131,7 +131,9 @@ int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen) {
     }
     if (ctx->stats) {
         static const char *stat_names[STAT_COUNT] = {"stat_p1_leaves", "stat_p1_inserts", "stat_p1_nodes",
-                                                     "stat_p2_leaves", "stat_p2_nodes", "stat_waves"};
+                                                     "stat_p2_leaves", "stat_p2_nodes", "stat_waves",
+                                                     "stat_lane_need", "stat_maxlane_need",
+                                                     "stat_parent_max"};
         unsigned long long hs[16];
         rc = check_hip(ctx, hipMemcpy(hs, ctx->stats, sizeof hs, hipMemcpyDeviceToHost), "stats read");
         if (rc) return rc;
@@ -292,6 +294,38 @@ int mhip_kernel(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out) {
     MHIP_NEED(q && out, "null argument");
     hipSetDevice(ctx->device);
     return kernel_eval(ctx, q, f64 ? 1 : 0, n, out);
+}
+
+int mhip_kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(dist && h && w, "null argument");
+    hipSetDevice(ctx->device);
+    return kernel_weights(ctx, dist, h, nq, k, w);
+}
+
+int mhip_kernel_average(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows,
+                        double *out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(nl && f && out, "null argument");
+    MHIP_NEED(nf_rows >= 0, "negative field length");
+    hipSetDevice(ctx->device);
+    return kernel_stat(ctx, nl, f, f_f64 ? 1 : 0, nf_rows, 1, 0, out);
+}
+
+int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows,
+                         int K, double *out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(nl && f && out, "null argument");
+    MHIP_NEED(nf_rows >= 0, "negative field length");
+    hipSetDevice(ctx->device);
+    return kernel_stat(ctx, nl, f, f_f64 ? 1 : 0, nf_rows, K, 1, out);
+}
+
+int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(vol && f && out, "null argument");
+    hipSetDevice(ctx->device);
+    return integrate(ctx, vol, f, f_f64 ? 1 : 0, n, K, out);
 }
 
 }  // extern "C"

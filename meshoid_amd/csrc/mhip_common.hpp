@@ -131,8 +131,12 @@ enum : int {
     STAT_P2_LEAVES,       // leaves scanned in pass 2
     STAT_P2_NODES,        // recorded leaves re-scanned for members (main pass)
     STAT_WAVES,           // wavefronts of pass 1
+    STAT_LANE_NEED,       // leaves under a leaf-parent that a lane itself needs (sum over lanes)
+    STAT_MAXLANE_NEED,    // per wavefront, the largest such count of one lane (sum over waves)
+    STAT_PARENT_MAX,      // per leaf-parent expansion, the largest count of one lane (sum)
     STAT_COUNT
 };
+static_assert(STAT_COUNT <= 16, "the work-counter buffer holds 16 entries");
 }
 
 namespace mhip {

@@ -128,19 +128,6 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
     if (a.status) a.status[row] = st;
 }
 
-template <typename FT>
-__global__ __launch_bounds__(256) void k_permute_fields(const FT *__restrict__ f, int64_t nf_rows, int K,
-                                                       const uint32_t *__restrict__ perm, int64_t n,
-                                                       int64_t npad, FT *__restrict__ fs) {
-    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (p >= npad) return;
-    const uint32_t o = p < n ? perm[p] : 0xffffffffu;
-    for (int c = 0; c < K; c++) {
-        FT v = (o < nf_rows) ? f[(int64_t)o * K + c] : (FT)NAN;
-        fs[(int64_t)c * npad + p] = v;
-    }
-}
-
 template <int DIM, int ORDER, typename FT>
 static void launch_apply(const ApplyArgs &a, dim3 g, hipStream_t s) {
     if (ORDER == 1)

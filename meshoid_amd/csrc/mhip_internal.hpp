@@ -49,4 +49,25 @@ int hsml_dists(mhip_ctx *ctx, const void *d, int d_f64, int64_t n, int k, int di
 
 int kernel_eval(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out);
 
+// kernel-weighted neighbour statistics (mhip_kops.hip)
+int kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w);
+int kernel_stat(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows, int K,
+                int which, double *out);
+int integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
+
+// field rows (original order, K components each) -> sorted-space columns
+// [K][npad]; rows beyond nf_rows and the padding read NaN
+template <typename FT>
+__global__ __launch_bounds__(256) void k_permute_fields(const FT *__restrict__ f, int64_t nf_rows, int K,
+                                                       const uint32_t *__restrict__ perm, int64_t n,
+                                                       int64_t npad, FT *__restrict__ fs) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= npad) return;
+    const uint32_t o = p < n ? perm[p] : 0xffffffffu;
+    for (int c = 0; c < K; c++) {
+        FT v = (o < nf_rows) ? f[(int64_t)o * K + c] : (FT)NAN;
+        fs[(int64_t)c * npad + p] = v;
+    }
+}
+
 }  // namespace mhip

@@ -465,6 +465,7 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     for (int j = 0; j < KCAP; j++) A[j] = j < e ? -1.0f : tau0;
     float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
     uint32_t n_ins = 0, n_leaf = 0, n_nodes = 0;
+    uint32_t need_lane = 0, parent_max = 0;  // work counters (stats only)
 
     // (1) the leaves holding this wave's own queries first (they fill every
     // list with a nearby bound), then (2) a wave-uniform depth-first traversal
@@ -526,6 +527,17 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
                         off = base + c;
                         ms.put(lvl, expand_mask<PER>(t, qb, y, bound));
                         n_nodes += FANOUT;
+                        if (a.stats && lvl + 1 == depth) {
+                            // work counters: leaves this lane itself needs
+                            int pc = 0;
+                            for (int cc = 0; cc < FANOUT; cc++) {
+                                const int64_t lf = (int64_t)(FANOUT * y + 1 + cc - first_leaf);
+                                pc += box_gap2_u<PER>(t, qb, FANOUT * y + 1 + cc) <= bound &&
+                                      !(lf >= l0 && lf <= l1) && lf < t.nleaf;
+                            }
+                            need_lane += pc;
+                            parent_max += wave_max(pc);
+                        }
                         continue;
                     }
                     const int64_t cand = (int64_t)(y - first_leaf);
@@ -579,6 +591,14 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         atomicAdd(a.stats + STAT_P1_INSERTS, (unsigned long long)n_ins);
         atomicAdd(a.stats + STAT_P2_LEAVES, (unsigned long long)nwl);
         atomicAdd(a.stats + STAT_WAVES, 1ull);
+        atomicAdd(a.stats + STAT_PARENT_MAX, (unsigned long long)parent_max);
+    }
+    if (a.stats) {
+        const uint32_t tot = wave_sum(need_lane), mx = (uint32_t)wave_max((int)need_lane);
+        if (lane == 0) {
+            atomicAdd(a.stats + STAT_LANE_NEED, (unsigned long long)tot);
+            atomicAdd(a.stats + STAT_MAXLANE_NEED, (unsigned long long)mx);
+        }
     }
 
     // (3) epilogue

@@ -233,3 +233,57 @@ def kernel_dev(q):
     ctx = context(q.device)
     raise_for(lib.mhip_kernel(ctx, ptr(q), int(q.dtype == torch.float64), q.numel(), ptr(out)), ctx)
     return out
+
+
+def kernel_weights_dev(dist: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
+    """W(d/h) normalised per row (Meshoid.get_kernel_weights), float64 (nq, k)."""
+    _need(dist, (torch.float32,), "dist")
+    _need(h, (torch.float32,), "h")
+    nq, k = dist.shape
+    if h.shape[0] != nq:
+        raise ValueError("h must have one entry per row of dist")
+    w = torch.empty((nq, k), dtype=torch.float64, device=dist.device)
+    ctx = context(dist.device)
+    raise_for(lib.mhip_kernel_weights(ctx, ptr(dist), ptr(h), nq, k, ptr(w)), ctx)
+    return w
+
+
+def kernel_average(nl: NeighborList, f: torch.Tensor) -> torch.Tensor:
+    """Kernel-weighted neighbour average of a scalar field (Meshoid.KernelAverage):
+    float64 (nq,)."""
+    _need(f, (torch.float32, torch.float64), "f")
+    if f.dim() != 1:
+        raise ValueError("KernelAverage takes a scalar field of shape (N,)")
+    out = torch.empty(nl.nq, dtype=torch.float64, device=f.device)
+    ctx = context(f.device)
+    raise_for(lib.mhip_kernel_average(ctx, nl.h, ptr(f), int(f.dtype == torch.float64), f.shape[0],
+                                      ptr(out)), ctx)
+    return out
+
+
+def kernel_variance(nl: NeighborList, f: torch.Tensor) -> torch.Tensor:
+    """Population standard deviation of f over each row's neighbours
+    (Meshoid.KernelVariance): float64 (nq, K) for f of shape (N, K)."""
+    _need(f, (torch.float32, torch.float64), "f")
+    f2 = f.reshape(f.shape[0], -1).contiguous() if f.dim() != 2 else f
+    K = f2.shape[1]
+    out = torch.empty((nl.nq, K), dtype=torch.float64, device=f.device)
+    ctx = context(f.device)
+    raise_for(lib.mhip_kernel_variance(ctx, nl.h, ptr(f2), int(f.dtype == torch.float64), f2.shape[0],
+                                       K, ptr(out)), ctx)
+    return out
+
+
+def integrate(vol: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
+    """sum_i vol_i f_i per component (Meshoid.Integrate): float64 (K,)."""
+    _need(vol, (torch.float32,), "vol")
+    _need(f, (torch.float32, torch.float64), "f")
+    f2 = f.reshape(f.shape[0], -1).contiguous()
+    if f2.shape[0] != vol.shape[0]:
+        raise ValueError("f and vol must have the same number of rows")
+    K = f2.shape[1]
+    out = torch.empty(K, dtype=torch.float64, device=f.device)
+    ctx = context(f.device)
+    raise_for(lib.mhip_integrate(ctx, ptr(vol), ptr(f2), int(f.dtype == torch.float64), f2.shape[0], K,
+                                 ptr(out)), ctx)
+    return out

@@ -209,7 +209,71 @@ class Meshoid:
                      "status": out["status"]}
 
     def get_kernel_weights(self):
-        raise NotImplementedError("get_kernel_weights is outside the GPU hot path")
+        """W(r_ij / h_i) normalised over each row's neighbours
+        (Meshoid.py:172-177): (Nmask, k) float64, columns as ``ngbdist``."""
+        if self.ngbdist is None or self.kernel_radius is None:
+            self.TreeUpdate()
+        if self._nl is not None and "ngbdist" not in self._dev:
+            self._export()
+        dist = self._dev.get("ngbdist")
+        h = self._dev.get("kernel_radius")
+        dev = self._pos_d.device
+        if dist is None:  # lists or radii assigned by the caller
+            dist = torch.from_numpy(np.ascontiguousarray(self.ngbdist, dtype=np.float32)).to(dev)
+        if h is None or "kernel_radius" in self._np:
+            h = torch.from_numpy(np.ascontiguousarray(self.kernel_radius, dtype=np.float32)).to(dev)
+        return engine.kernel_weights_dev(dist.contiguous(), h.contiguous()).cpu().numpy()
+
+    def _need_device_list(self, what):
+        if self._nl is None:
+            raise NotImplementedError(f"{what} runs on the GPU neighbour list of TreeUpdate(); "
+                                      "lists assigned by the caller are not supported")
+
+    def KernelAverage(self, f):
+        """Kernel-weighted average of a scalar field over each row's
+        neighbours (Meshoid.py:369-381): (Nmask,) float64."""
+        if self.ngbdist is None or self.kernel_radius is None:  # via get_kernel_weights
+            self.TreeUpdate()
+        self._need_device_list("KernelAverage")
+        is_t = isinstance(f, torch.Tensor)
+        if len(f.shape) != 1:
+            # einsum("ij,ij->i") of the reference rejects extra axes
+            raise ValueError("operands could not be broadcast: KernelAverage takes f of shape (N,)")
+        fd, _, _ = self._field(f)
+        out = engine.kernel_average(self._nl, fd.reshape(-1).contiguous())
+        return out if is_t else out.cpu().numpy()
+
+    def KernelVariance(self, f):
+        """Standard deviation of f over each row's neighbours
+        (Meshoid.py:352-367): (Nmask, ...)."""
+        if self.ngb is None:
+            self.TreeUpdate()
+        self._need_device_list("KernelVariance")
+        fd, tail, is_t = self._field(f)
+        out = engine.kernel_variance(self._nl, fd).reshape((self.Nmask,) + tuple(tail))
+        if is_t:
+            return out.to(f.dtype) if f.dtype in (torch.float32, torch.float64) else out
+        dt = np.asarray(f).dtype
+        return out.cpu().numpy().astype(dt if dt in (np.float32, np.float64) else np.float64)
+
+    def Integrate(self, f):
+        """Volume integral sum_i vol_i f_i over the masked particles
+        (Meshoid.py:333-350): shape f.shape[1:], float64."""
+        if self.kernel_radius is None:
+            self.TreeUpdate()
+        elif self.vol is None:
+            self.vol = self.volnorm * self.kernel_radius**self.dim
+        dev = self._pos_d.device
+        vol = self._dev.get("vol") if "vol" not in self._np else None
+        if vol is None:
+            vol = torch.from_numpy(np.ascontiguousarray(self.vol, dtype=np.float32)).to(dev)
+        is_t = isinstance(f, torch.Tensor)
+        tail = tuple(f.shape[1:])
+        fd = as_device(f)
+        if not self._identity_mask:
+            fd = fd[torch.as_tensor(self.particle_mask, device=dev)]
+        out = engine.integrate(vol.contiguous(), fd.reshape(fd.shape[0], -1).contiguous()).reshape(tail)
+        return out if is_t else out.cpu().numpy()
 
     def Volume(self):
         return self.vol

@@ -7,6 +7,8 @@
   optionally a uniform background fraction.  About six decades of density.
 * :func:`sin3`       -- f = sin(kx) sin(ky) sin(kz) and its exact gradient.
 * :func:`rotation_field` -- v = Omega x (r - c) + N(0, noise^2).
+* :func:`grf`        -- Gaussian random field with P(k) ~ k^slope on a periodic
+  grid, seeded, trilinearly interpolated to the particles.
 """
 from __future__ import annotations
 
@@ -58,3 +60,47 @@ def rotation_field(pos, omega=(0.0, 0.0, 1.0), center=50.0, noise=0.1, seed=0):
     if noise:
         v = v + noise * np.random.default_rng(seed).standard_normal(v.shape)
     return v
+
+
+def grf_grid(ngrid=256, box=100.0, slope=-3.0, seed=0):
+    """Periodic Gaussian random field on an ngrid^3 grid over [0, box)^3 with
+    power spectrum P(k) ~ k^slope (the k = 0 mode removed), unit variance."""
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((ngrid, ngrid, ngrid))
+    wk = np.fft.rfftn(white)
+    kx = np.fft.fftfreq(ngrid, d=box / ngrid) * 2 * np.pi
+    kz = np.fft.rfftfreq(ngrid, d=box / ngrid) * 2 * np.pi
+    k2 = kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2
+    amp = np.zeros_like(k2)
+    nz = k2 > 0
+    amp[nz] = k2[nz] ** (slope / 4.0)  # sqrt(P(k)) with P ~ |k|^slope
+    field = np.fft.irfftn(wk * amp, s=white.shape)
+    field -= field.mean()
+    return field / field.std()
+
+
+def trilinear_periodic(grid, pos, box=100.0):
+    """Trilinear interpolation of a periodic grid over [0, box)^3 at pos."""
+    ng = grid.shape[0]
+    u = np.mod(np.asarray(pos, dtype=np.float64), box) * (ng / box)
+    i0 = np.floor(u).astype(np.int64)
+    t = u - i0
+    i0 %= ng
+    i1 = (i0 + 1) % ng
+    out = np.zeros(len(u))
+    for dx in (0, 1):
+        wx = t[:, 0] if dx else 1 - t[:, 0]
+        ix = i1[:, 0] if dx else i0[:, 0]
+        for dy in (0, 1):
+            wy = t[:, 1] if dy else 1 - t[:, 1]
+            iy = i1[:, 1] if dy else i0[:, 1]
+            for dz in (0, 1):
+                wz = t[:, 2] if dz else 1 - t[:, 2]
+                iz = i1[:, 2] if dz else i0[:, 2]
+                out += wx * wy * wz * grid[ix, iy, iz]
+    return out
+
+
+def grf(pos, ngrid=256, box=100.0, slope=-3.0, seed=0):
+    """GRF values at the particles (float64)."""
+    return trilinear_periodic(grf_grid(ngrid, box, slope, seed), pos, box)

@@ -16,6 +16,9 @@ meshoid source tree):
   ``src/meshoid/Meshoid.py:258,268,291,295``.
 * :class:`OracleMeshoid`     -- the ``Meshoid`` state machine for the hot-path
   methods (``Meshoid.py:15-105,110-170,237-331``).
+* :func:`kernel_weights`, :func:`kernel_average`, :func:`kernel_variance`,
+  :func:`integrate` -- ``get_kernel_weights``, ``KernelAverage``,
+  ``KernelVariance`` and ``Integrate``, ``src/meshoid/Meshoid.py:172-177,333-381``.
 
 The numerical core is ``meshoid_oracle.c`` (strict FP64, no FMA contraction),
 built into ``liboracle.so`` next to this file.
@@ -105,6 +108,28 @@ def hsml_iter(neighbor_dists, dim=3, error_norm=1e-6, max_iter=1000, return_iter
 def kernel(q):
     q = np.asarray(q, dtype=np.float64)
     return np.where(q <= 0.5, 1 - 6 * q**2 + 6 * q**3, np.where(q <= 1.0, 2 * (1 - q) ** 3, 0.0))
+
+
+def kernel_weights(ngbdist, kernel_radius):
+    """Kernel(ngbdist / h) normalised per row (Meshoid.py:172-177)."""
+    K = kernel(np.asarray(ngbdist, np.float64) / np.asarray(kernel_radius, np.float64)[:, None])
+    return K / np.sum(K, axis=1)[:, None]
+
+
+def kernel_average(ngbdist, kernel_radius, ngb, f):
+    """einsum("ij,ij->i", weights, f[ngb]) (Meshoid.py:369-381), scalar f."""
+    return np.einsum("ij,ij->i", kernel_weights(ngbdist, kernel_radius),
+                     np.asarray(f, np.float64)[np.asarray(ngb)])
+
+
+def kernel_variance(ngb, f):
+    """np.std(f[ngb], axis=1) (Meshoid.py:352-367), float64."""
+    return np.std(np.asarray(f, np.float64)[np.asarray(ngb)], axis=1)
+
+
+def integrate(vol, f):
+    """einsum("i,i...->...", vol, f) over the masked rows (Meshoid.py:333-350)."""
+    return np.einsum("i,i...->...", np.asarray(vol, np.float64), np.asarray(f, np.float64))
 
 
 def num_derivs(dim, order):
