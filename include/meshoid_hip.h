@@ -44,6 +44,7 @@ extern "C" {
 typedef struct mhip_ctx mhip_ctx;
 typedef struct mhip_index mhip_index;
 typedef struct mhip_nlist mhip_nlist;
+typedef struct mhip_batch mhip_batch;
 
 /* library version, e.g. 10000 for 1.0.0 */
 int mhip_version(void);
@@ -194,6 +195,27 @@ int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int
  * out float64[K].  Deterministic two-level float64 reduction.
  */
 int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
+
+/*
+ * Batched form (no counterpart in the reference, which runs one Meshoid at a
+ * time: Meshoid.py:15-170 then D at :237-268 per snapshot).  nsnap particle
+ * sets of n[s] float32 positions (n[s], 3) each, with des_ngb k[s] and K
+ * float32 fields (n[s], K); per snapshot: h, rho float32[n[s]], the weighted
+ * gradient of every field grad float32[n[s] * K * 3] (as mhip_apply_D, order
+ * 1) and status int32[n[s]] (kNN bits OR derivative bits).  Snapshots are
+ * spread over nbranch streams; with use_graph the whole batch (fork/join of
+ * the branches + one contraction launch over all snapshots) is captured once
+ * into a HIP graph and replayed (recaptured when the buffers change).  The
+ * first run is eager and times the contraction's field tile (2/4/8/16 fields
+ * per pass); mhip_batch_info reports the choice.  Non-periodic sets only.
+ */
+int mhip_batch_create(mhip_ctx *ctx, int nsnap, const int64_t *n, const int *k, int K, int nbranch,
+                      mhip_batch **out);
+int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const float *const *f,
+                   float *const *h, float *const *rho, float *const *grad, int32_t *const *status,
+                   int use_graph);
+int mhip_batch_info(const mhip_batch *b, int *field_tile, float *tune_ms4, int *nbranch);
+int mhip_batch_destroy(mhip_batch *b);
 
 /*
  * Learned smoothing-length seed (no counterpart in the reference, whose

@@ -28,6 +28,7 @@ EXPORTED = [
     "mhip_hsml_iter", "mhip_kernel", "mhip_prof_enable", "mhip_prof_read", "mhip_seed_features",
     "mhip_seed_predict", "mhip_mlp_train_step", "mhip_mlp_pack", "mhip_mlp_blob_bytes",
     "mhip_kernel_weights", "mhip_kernel_average", "mhip_kernel_variance", "mhip_integrate",
+    "mhip_batch_create", "mhip_batch_run", "mhip_batch_info", "mhip_batch_destroy",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -74,6 +75,10 @@ _SIGS = {
     "mhip_kernel_average": ([_p, _p, _p, _i, _i64, _p], _i),
     "mhip_kernel_variance": ([_p, _p, _p, _i, _i64, _i, _p], _i),
     "mhip_integrate": ([_p, _p, _p, _i, _i64, _i, _p], _i),
+    "mhip_batch_create": ([_p, _i, _p, _p, _i, _i, _pp], _i),
+    "mhip_batch_run": ([_p, _p, _p, _p, _p, _p, _p, _p, _i], _i),
+    "mhip_batch_info": ([_p, _p, _p, _p], _i),
+    "mhip_batch_destroy": ([_p], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     _fn = getattr(lib, _name)

@@ -1,0 +1,320 @@
+// Batched form: many independent particle sets ("snapshots") of uneven size,
+// each with its own des_ngb, and K fields per particle, through one HIP graph.
+//
+// No counterpart in the reference, which handles one Meshoid at a time
+// (src/meshoid/Meshoid.py:15-170 per snapshot, then D at :237-268).  Per
+// snapshot the work is the same as the single-set path (index build, kNN +
+// smoothing length + density, field permutation); the derivative contraction
+// of all snapshots is one launch over a device table of per-snapshot
+// arguments.
+//
+// Execution: snapshots are spread over nbranch streams (largest first, each
+// to the branch with the least work so far); the main stream forks to the
+// branches with events and joins them before the batched contraction.  A run
+// with use_graph captures that fork/join sequence once (after one eager run
+// has sized every buffer) and replays it with hipGraphLaunch; the graph is
+// recaptured when the caller passes different buffers.  Index builds run in
+// deferred-check mode (no host synchronisation inside the graph); their
+// bbox / NaN records are validated after each run.
+//
+// The contraction's field tile (fields per pass over the neighbours: 2, 4, 8
+// or 16) is chosen by timing every candidate on the first eager run.
+#include <algorithm>
+#include <new>
+#include <numeric>
+#include <string>
+
+#include "../../include/meshoid_hip.h"
+#include "mhip_internal.hpp"
+
+struct mhip_batch {
+    int nsnap = 0, K = 0, nbranch = 0;
+    std::vector<int64_t> n;
+    std::vector<int> k, branch_of;
+    std::vector<mhip_index *> ix;
+    std::vector<mhip_nlist *> nl;
+    std::vector<mhip_ctx *> br;
+    std::vector<hipStream_t> streams;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    float *recs = nullptr;              // [nsnap][12] index records
+    float *fsorted = nullptr;           // per snapshot [K][npad] sorted-space fields
+    std::vector<int64_t> foff;          // offsets into fsorted (floats)
+    mhip::ApplyArgs *tab_d = nullptr;   // [nsnap]
+    int64_t *blk0_d = nullptr;          // [nsnap + 1]
+    int64_t nblocks = 0;
+    int kc = 0;                         // chosen field tile
+    float tune_ms[4] = {0, 0, 0, 0};    // per candidate 2, 4, 8, 16
+    bool warmed = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void *> key;      // buffers the graph was captured with
+};
+
+namespace mhip {
+
+static const int KC_CAND[4] = {2, 4, 8, 16};
+
+static int64_t npad_of(int64_t n) { return (n + LEAF - 1) / LEAF * LEAF; }
+
+// per-snapshot work on its branch: index, kNN + h + rho, field permutation
+static int enqueue_snapshot(mhip_batch *b, int s, const float *pos, const float *f, float *h, float *rho,
+                            int32_t *status) {
+    mhip_ctx *c = b->br[b->branch_of[s]];
+    int rc = index_build(c, pos, 0, b->n[s], 3, -1.0, b->ix[s], b->recs + 12 * s);
+    if (rc) return rc;
+    rc = knn_run(c, b->ix[s], nullptr, b->n[s], b->k[s], nullptr, nullptr, nullptr, h, rho, nullptr, status,
+                 0, b->nl[s]);
+    if (rc) return rc;
+    const int64_t np = b->ix[s]->npad;
+    hipLaunchKernelGGL(k_permute_fields<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, f,
+                       b->n[s], b->K, b->ix[s]->perm, b->n[s], np, b->fsorted + b->foff[s]);
+    return check_hip(c, hipGetLastError(), "batch field permutation");
+}
+
+// fork to the branches, per-snapshot work, join (the neighbour lists and
+// indices get their device buffers here on the first run)
+static int enqueue_snapshots(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const float *const *f,
+                             float *const *h, float *const *rho, int32_t *const *status) {
+    int rc = check_hip(ctx, hipEventRecord(b->fork, ctx->stream), "batch fork");
+    if (rc) return rc;
+    for (int i = 0; i < b->nbranch; i++)
+        if ((rc = check_hip(ctx, hipStreamWaitEvent(b->streams[i], b->fork, 0), "batch fork wait"))) return rc;
+    for (int s = 0; s < b->nsnap; s++) {
+        if ((rc = enqueue_snapshot(b, s, pos[s], f[s], h[s], rho[s], status[s]))) {
+            ctx->last_error = b->br[b->branch_of[s]]->last_error;
+            return rc;
+        }
+    }
+    for (int i = 0; i < b->nbranch; i++) {
+        if ((rc = check_hip(ctx, hipEventRecord(b->join[i], b->streams[i]), "batch join"))) return rc;
+        if ((rc = check_hip(ctx, hipStreamWaitEvent(ctx->stream, b->join[i], 0), "batch join wait"))) return rc;
+    }
+    return 0;
+}
+
+static int upload_table(mhip_ctx *ctx, mhip_batch *b, float *const *grad, int32_t *const *status) {
+    std::vector<ApplyArgs> tab(b->nsnap);
+    std::vector<int64_t> blk0(b->nsnap + 1, 0);
+    for (int s = 0; s < b->nsnap; s++) {
+        apply_args_fill(b->nl[s], b->fsorted + b->foff[s], b->K, grad[s], status[s], 1, &tab[s]);
+        blk0[s + 1] = blk0[s] + (b->n[s] + 255) / 256;
+    }
+    b->nblocks = blk0[b->nsnap];
+    int rc = check_hip(ctx, hipMemcpy(b->tab_d, tab.data(), tab.size() * sizeof(ApplyArgs), hipMemcpyHostToDevice),
+                       "batch table upload");
+    if (rc) return rc;
+    return check_hip(ctx, hipMemcpy(b->blk0_d, blk0.data(), blk0.size() * sizeof(int64_t), hipMemcpyHostToDevice),
+                     "batch table upload");
+}
+
+static int validate(mhip_ctx *ctx, mhip_batch *b) {
+    std::vector<float> r((size_t)b->nsnap * 12);
+    int rc = check_hip(ctx, hipMemcpyAsync(r.data(), b->recs, r.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                           ctx->stream),
+                       "batch record readback");
+    if (rc) return rc;
+    if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch run"))) return rc;
+    for (int s = 0; s < b->nsnap; s++)
+        if (r[12 * s + 6] != 0.f) return fail(ctx, MHIP_ERR_VALUE, "positions contain NaN (snapshot " + std::to_string(s) + ")");
+    return 0;
+}
+
+static void drop_graph(mhip_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->key.clear();
+}
+
+}  // namespace mhip
+
+using namespace mhip;
+
+extern "C" {
+
+int mhip_batch_create(mhip_ctx *ctx, int nsnap, const int64_t *n, const int *k, int K, int nbranch,
+                      mhip_batch **out) {
+    if (!ctx || !out) return MHIP_ERR_VALUE;
+    *out = nullptr;
+    if (nsnap < 1 || !n || !k) return fail(ctx, MHIP_ERR_VALUE, "batch needs at least one snapshot");
+    if (K < 1) return fail(ctx, MHIP_ERR_VALUE, "no field components");
+    if (nbranch < 1) nbranch = 1;
+    nbranch = std::min(nbranch, nsnap);
+    for (int s = 0; s < nsnap; s++) {
+        if (k[s] < 2 || k[s] > 128) return fail(ctx, MHIP_ERR_UNSUPPORTED, "des_ngb must be in [2, 128]");
+        if (n[s] < k[s]) return fail(ctx, MHIP_ERR_VALUE, "des_ngb exceeds the number of particles");
+    }
+    hipSetDevice(ctx->device);
+    mhip_batch *b = new (std::nothrow) mhip_batch();
+    if (!b) return MHIP_ERR_HIP;
+    b->nsnap = nsnap;
+    b->K = K;
+    b->nbranch = nbranch;
+    b->n.assign(n, n + nsnap);
+    b->k.assign(k, k + nsnap);
+    // largest snapshots first, each to the least loaded branch
+    std::vector<int> order(nsnap);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+    std::vector<double> load(nbranch, 0.0);
+    b->branch_of.assign(nsnap, 0);
+    for (int s : order) {
+        const int i = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->branch_of[s] = i;
+        load[i] += (double)n[s] * k[s];
+    }
+    int rc = 0;
+    auto bail = [&](int code) {
+        mhip_batch_destroy(b);
+        return code;
+    };
+    for (int i = 0; i < nbranch; i++) {
+        hipStream_t st;
+        if ((rc = check_hip(ctx, hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "batch stream"))) return bail(rc);
+        b->streams.push_back(st);
+        mhip_ctx *c = nullptr;
+        if (mhip_ctx_create(ctx->device, st, &c) != MHIP_OK) return bail(fail(ctx, MHIP_ERR_HIP, "batch context"));
+        b->br.push_back(c);
+        hipEvent_t e;
+        if ((rc = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "batch event"))) return bail(rc);
+        b->join.push_back(e);
+    }
+    if ((rc = check_hip(ctx, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch event"))) return bail(rc);
+    b->foff.assign(nsnap + 1, 0);
+    for (int s = 0; s < nsnap; s++) b->foff[s + 1] = b->foff[s] + (int64_t)K * npad_of(n[s]);
+    if ((rc = check_hip(ctx, hipMalloc((void **)&b->fsorted, b->foff[nsnap] * sizeof(float)), "hipMalloc (batch fields)")))
+        return bail(rc);
+    if ((rc = check_hip(ctx, hipMalloc((void **)&b->recs, (size_t)nsnap * 12 * sizeof(float)), "hipMalloc (batch)")))
+        return bail(rc);
+    if ((rc = check_hip(ctx, hipMalloc((void **)&b->tab_d, (size_t)nsnap * sizeof(ApplyArgs)), "hipMalloc (batch)")))
+        return bail(rc);
+    if ((rc = check_hip(ctx, hipMalloc((void **)&b->blk0_d, (size_t)(nsnap + 1) * sizeof(int64_t)), "hipMalloc (batch)")))
+        return bail(rc);
+    for (int s = 0; s < nsnap; s++) {
+        mhip_index *x = nullptr;
+        mhip_nlist *l = nullptr;
+        if (mhip_index_create(&x) != MHIP_OK || mhip_nlist_create(&l) != MHIP_OK) {
+            mhip_index_destroy(x);
+            mhip_nlist_destroy(l);
+            return bail(fail(ctx, MHIP_ERR_HIP, "batch index"));
+        }
+        b->ix.push_back(x);
+        b->nl.push_back(l);
+    }
+    *out = b;
+    return MHIP_OK;
+}
+
+int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const float *const *f,
+                   float *const *h, float *const *rho, float *const *grad, int32_t *const *status,
+                   int use_graph) {
+    if (!ctx || !b) return MHIP_ERR_VALUE;
+    if (!pos || !f || !h || !rho || !grad || !status) return fail(ctx, MHIP_ERR_VALUE, "null argument");
+    for (int s = 0; s < b->nsnap; s++)
+        if (!pos[s] || !f[s] || !h[s] || !rho[s] || !grad[s] || !status[s])
+            return fail(ctx, MHIP_ERR_VALUE, "null snapshot buffer");
+    hipSetDevice(ctx->device);
+    std::vector<const void *> key;
+    for (int s = 0; s < b->nsnap; s++) {
+        const void *p[6] = {pos[s], f[s], h[s], rho[s], grad[s], status[s]};
+        key.insert(key.end(), p, p + 6);
+    }
+    int rc = 0;
+    const int prof = ctx->prof;
+    if (!b->warmed || !use_graph) {
+        // eager run: sizes every buffer; the first one also picks the field
+        // tile.  The argument table holds the lists' device pointers, which
+        // exist once the snapshots are enqueued.
+        if ((rc = enqueue_snapshots(ctx, b, pos, f, h, rho, status))) return rc;
+        if ((rc = upload_table(ctx, b, grad, status))) return rc;
+        if (!b->kc) {
+            if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch warm-up"))) return rc;
+            hipEvent_t e0, e1;
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            float best = 0.f;
+            for (int c = 0; c < 4; c++) {
+                float ms = 0.f;
+                for (int rep = 0; rep < 3; rep++) {
+                    (void)hipEventRecord(e0, ctx->stream);
+                    rc = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, KC_CAND[c]);
+                    (void)hipEventRecord(e1, ctx->stream);
+                    if (rc) break;
+                    (void)hipEventSynchronize(e1);
+                    float t = 0.f;
+                    (void)hipEventElapsedTime(&t, e0, e1);
+                    if (rep > 0) ms += t / 2;
+                }
+                if (rc) break;
+                b->tune_ms[c] = ms;
+                if (!b->kc || ms < best) {
+                    b->kc = KC_CAND[c];
+                    best = ms;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (rc) return rc;
+        }
+        if ((rc = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, b->kc))) return rc;
+        b->warmed = true;
+        return validate(ctx, b);
+    }
+    if (!b->exec || key != b->key) {
+        drop_graph(b);
+        if ((rc = upload_table(ctx, b, grad, status))) return rc;
+        ctx->prof = 0;  // no timing events inside the graph
+        rc = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "batch capture");
+        if (!rc) {
+            int rce = enqueue_snapshots(ctx, b, pos, f, h, rho, status);
+            if (!rce) rce = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, b->kc);
+            hipGraph_t g = nullptr;
+            const int rc2 = check_hip(ctx, hipStreamEndCapture(ctx->stream, &g), "batch capture end");
+            rc = rce ? rce : rc2;
+            if (!rc) {
+                b->graph = g;
+                rc = check_hip(ctx, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+            } else if (g) {
+                (void)hipGraphDestroy(g);
+            }
+        }
+        ctx->prof = prof;
+        if (rc) {
+            drop_graph(b);
+            return rc;
+        }
+        b->key = key;
+    }
+    if ((rc = check_hip(ctx, hipGraphLaunch(b->exec, ctx->stream), "graph launch"))) return rc;
+    return validate(ctx, b);
+}
+
+int mhip_batch_info(const mhip_batch *b, int *kc, float *tune_ms, int *nbranch) {
+    if (!b) return MHIP_ERR_VALUE;
+    if (kc) *kc = b->kc;
+    if (tune_ms)
+        for (int c = 0; c < 4; c++) tune_ms[c] = b->tune_ms[c];
+    if (nbranch) *nbranch = b->nbranch;
+    return MHIP_OK;
+}
+
+int mhip_batch_destroy(mhip_batch *b) {
+    if (!b) return MHIP_OK;
+    drop_graph(b);
+    for (auto c : b->br) mhip_ctx_destroy(c);
+    for (auto s : b->streams) (void)hipStreamDestroy(s);
+    for (auto e : b->join) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (auto x : b->ix) mhip_index_destroy(x);
+    for (auto l : b->nl) mhip_nlist_destroy(l);
+    if (b->fsorted) (void)hipFree(b->fsorted);
+    if (b->recs) (void)hipFree(b->recs);
+    if (b->tab_d) (void)hipFree(b->tab_d);
+    if (b->blk0_d) (void)hipFree(b->blk0_d);
+    delete b;
+    return MHIP_OK;
+}
+
+}  // extern "C"

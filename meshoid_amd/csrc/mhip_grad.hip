@@ -21,29 +21,12 @@ namespace mhip {
 // ---------------------------------------------------------------------------
 // fused operator over the neighbour list
 // ---------------------------------------------------------------------------
-struct ApplyArgs {
-    const float *sx, *sy, *sz, *lx, *ly, *lz;
-    float box, inv_box;
-    int per, p64;
-    const uint32_t *nl_idx;
-    const uint32_t *qsorted, *qrow, *perm;
-    const float *h;
-    int64_t nq, npad;
-    int k;
-    const void *fs;  // [K][npad] sorted-space fields
-    int K;
-    int weighted;
-    float *out1, *out2;  // [row][K][DIM], [row][K][ND-DIM]
-    int32_t *status;     // per row, may be null
-};
 
 
+// one query slot s of the list: KC fields per pass over the neighbours
 template <int DIM, int ORDER, typename FT, int KC>
-__global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
+MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s) {
     constexpr int ND = NDer<DIM, ORDER>::ND, NE = NDer<DIM, ORDER>::NE;
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t s = (int64_t)b * blockDim.x + threadIdx.x;
-    if (s >= a.nq) return;
     const uint32_t qp = a.qsorted ? a.qsorted[s] : (uint32_t)s;
     const uint32_t row = a.qrow ? a.qrow[s] : a.perm[s];
     // host particle in float64: hi + lo reproduces float64 inputs to 2^-48
@@ -125,7 +108,77 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
             }
         }
     }
-    if (a.status) a.status[row] = st;
+    if (a.status) a.status[row] = a.or_status ? (a.status[row] | st) : st;
+}
+
+template <int DIM, int ORDER, typename FT, int KC>
+__global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t s = (int64_t)b * blockDim.x + threadIdx.x;
+    if (s >= a.nq) return;
+    apply_row<DIM, ORDER, FT, KC>(a, s);
+}
+
+// every snapshot of a batch in one launch: blk0[i] is the first block of
+// snapshot i (blk0[nsnap] = grid size); each block finds its snapshot's
+// arguments in the device table
+template <int KC>
+__global__ __launch_bounds__(256) void k_apply_batch(const ApplyArgs *__restrict__ tab,
+                                                     const int64_t *__restrict__ blk0, int nsnap) {
+    const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    int lo = 0, hi = nsnap;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (blk0[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const ApplyArgs a = tab[lo];
+    const int64_t s = (b - blk0[lo]) * blockDim.x + threadIdx.x;
+    if (s >= a.nq) return;
+    apply_row<3, 1, float, KC>(a, s);
+}
+
+int apply_batch_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
+                       int64_t nblocks, int kc) {
+    const dim3 g((unsigned)nblocks), b(256);
+    switch (kc) {
+        case 2: hipLaunchKernelGGL(k_apply_batch<2>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        case 4: hipLaunchKernelGGL(k_apply_batch<4>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        case 8: hipLaunchKernelGGL(k_apply_batch<8>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        case 16: hipLaunchKernelGGL(k_apply_batch<16>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        default: return fail(ctx, -1, "unsupported field tile");
+    }
+    return check_hip(ctx, hipGetLastError(), "batched apply launch");
+}
+
+void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, int32_t *status,
+                     int or_status, ApplyArgs *a) {
+    const mhip_index *ix = nl->index;
+    a->sx = ix->sx;
+    a->sy = ix->sy;
+    a->sz = ix->sz;
+    a->lx = ix->lx;
+    a->ly = ix->ly;
+    a->lz = ix->lz;
+    a->per = ix->box > 0;
+    a->box = a->per ? (float)ix->box : 0.f;
+    a->inv_box = a->per ? (float)(1.0 / ix->box) : 0.f;
+    a->p64 = ix->pos_f64;
+    a->nl_idx = nl->idx;
+    a->qsorted = nl->masked ? nl->qsorted : nullptr;
+    a->qrow = nl->masked ? nl->qrow : nullptr;
+    a->perm = ix->perm;
+    a->h = nl->h;
+    a->nq = nl->nq;
+    a->npad = ix->npad;
+    a->k = nl->k;
+    a->fs = fs;
+    a->K = K;
+    a->weighted = 1;
+    a->out1 = out1;
+    a->out2 = nullptr;
+    a->status = status;
+    a->or_status = or_status;
 }
 
 template <int DIM, int ORDER, typename FT>
@@ -192,6 +245,7 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     a.out1 = out1;
     a.out2 = out2;
     a.status = status;
+    a.or_status = 0;
     prp.end();
     ProfRegion pra(ctx, order == 1 ? "apply_D1" : "apply_D2");
     const dim3 g((unsigned)((nl->nq + 255) / 256));

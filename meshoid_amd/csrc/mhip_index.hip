@@ -347,7 +347,7 @@ static int grow(mhip_ctx *ctx, void **p, size_t bytes) {
 }
 
 int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim, double box,
-                mhip_index *ix) {
+                mhip_index *ix, float *rec) {
     if (n < 1) return fail(ctx, -1, "index needs at least one particle");
     if (n >= (int64_t)0xffffffff) return fail(ctx, -4, "more than 2^32-2 particles");
     if (dim < 1 || dim > 3) return fail(ctx, -5, "dim must be 1, 2 or 3");
@@ -397,7 +397,7 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     uint64_t *k0 = (uint64_t *)(s + off_k0), *k1 = (uint64_t *)(s + off_k1);
     uint32_t *v0 = (uint32_t *)(s + off_v0), *v1 = (uint32_t *)(s + off_v1);
     uint32_t *counts = (uint32_t *)(s + off_c);
-    float *part = (float *)(s + off_bb), *bb = part + nbb * 8;
+    float *part = (float *)(s + off_bb), *bb = rec ? rec : part + nbb * 8;
     const int nb = (int)std::min<int64_t>(nbb, (n + 255) / 256);
     const dim3 g1((unsigned)((n + 255) / 256)), b256(256);
     ProfRegion pr_bb(ctx, "index_bbox");
@@ -444,6 +444,13 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     }
     pr_g.end();
     if ((rc = check_hip(ctx, hipGetLastError(), "index build launch"))) return rc;
+    if (rec) {
+        // deferred: the caller reads the record (bbox, NaN flag, frame) later;
+        // the host copies are unknown until then
+        for (int c = 0; c < 6; c++) ix->bbox[c] = NAN;
+        for (int c = 0; c < 4; c++) ix->frame[c] = NAN;
+        return 0;
+    }
     // bbox + nan flag back to the host: periodic range and NaN checks
     float hb[12];
     if ((rc = check_hip(ctx, hipMemcpyAsync(hb, bb, 12 * sizeof(float), hipMemcpyDeviceToHost,

@@ -17,9 +17,12 @@ int radix_sort_pairs(mhip_ctx *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, ui
                      uint32_t *counts, int64_t n, int bits, uint64_t **kout, uint32_t **vout);
 size_t radix_counts_words(int64_t n);
 
-// spatial index build (bbox -> keys -> sort -> reorder -> boxes)
+// spatial index build (bbox -> keys -> sort -> reorder -> boxes).  With rec
+// (device, 12 floats: bbox lo/hi, NaN flag, pad, frame) the build neither
+// synchronises nor checks: the record is left for the caller to validate
+// (float32 positions only, since the float64 box-test slack needs the bbox).
 int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim, double box,
-                mhip_index *ix);
+                mhip_index *ix, float *rec = nullptr);
 
 // query setup + pass1/pass2/resolve
 int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
@@ -48,6 +51,31 @@ int hsml_dists(mhip_ctx *ctx, const void *d, int d_f64, int64_t n, int k, int di
                double error_norm, double *h, int32_t *status);
 
 int kernel_eval(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out);
+
+// arguments of the fused derivative operator over one neighbour list
+struct ApplyArgs {
+    const float *sx, *sy, *sz, *lx, *ly, *lz;
+    float box, inv_box;
+    int per, p64;
+    const uint32_t *nl_idx;
+    const uint32_t *qsorted, *qrow, *perm;
+    const float *h;
+    int64_t nq, npad;
+    int k;
+    const void *fs;  // [K][npad] sorted-space fields
+    int K;
+    int weighted;
+    float *out1, *out2;  // [row][K][DIM], [row][K][ND-DIM]
+    int32_t *status;     // per row, may be null
+    int or_status;       // 1: OR the bits into status instead of storing them
+};
+
+// every snapshot of a batch in one launch (table and block prefix on the
+// device); kc = fields per pass over the neighbours (2, 4, 8 or 16)
+int apply_batch_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
+                       int64_t nblocks, int kc);
+void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, int32_t *status,
+                     int or_status, ApplyArgs *a);
 
 // kernel-weighted neighbour statistics (mhip_kops.hip)
 int kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w);
