@@ -84,7 +84,9 @@ int kernel_stat(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, i
 int integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
 
 // field rows (original order, K components each) -> sorted-space columns
-// [K][npad]; rows beyond nf_rows and the padding read NaN
+// [K][npad]: the j-th neighbours of a wave's 64 consecutive queries lie close
+// together in sorted order, so one field's gathers share cache lines; rows
+// beyond nf_rows and the padding read NaN
 template <typename FT>
 __global__ __launch_bounds__(256) void k_permute_fields(const FT *__restrict__ f, int64_t nf_rows, int K,
                                                        const uint32_t *__restrict__ perm, int64_t n,

@@ -104,3 +104,26 @@ def trilinear_periodic(grid, pos, box=100.0):
 def grf(pos, ngrid=256, box=100.0, slope=-3.0, seed=0):
     """GRF values at the particles (float64)."""
     return trilinear_periodic(grf_grid(ngrid, box, slope, seed), pos, box)
+
+
+def snapshot_batch(nsnap=64, K=32, seed=0, n_lo=16384, n_hi=1048576, box=100.0):
+    """Uneven particle sets for the batched form: N_s = round(exp(U[ln n_lo,
+    ln n_hi])), 8-128 clumps each (10% background), des_ngb alternating 32 /
+    64, and K float32 fields N(0,1) + a.x + x^T B x per set.  Returns a list
+    of (pos float32 (N_s, 3), fields float32 (N_s, K), des_ngb)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(nsnap):
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        pos = clustered(n, seed=seed * 1000 + s, nclump=int(rng.integers(8, 129)), box=box, background=0.1)
+        x = pos.astype(np.float64) / box
+        a = rng.standard_normal((3, K))
+        B = rng.standard_normal((K, 3, 3))
+        # x^T B x = sum_ij B_ij x_i x_j through the 6 quadratic monomials
+        q = np.stack([x[:, 0] ** 2, x[:, 1] ** 2, x[:, 2] ** 2, x[:, 0] * x[:, 1], x[:, 1] * x[:, 2],
+                      x[:, 0] * x[:, 2]], axis=1)
+        C = np.stack([B[:, 0, 0], B[:, 1, 1], B[:, 2, 2], B[:, 0, 1] + B[:, 1, 0], B[:, 1, 2] + B[:, 2, 1],
+                      B[:, 0, 2] + B[:, 2, 0]], axis=0)
+        f = rng.standard_normal((n, K), dtype=np.float32) + (x @ a + q @ C).astype(np.float32)
+        out.append((pos, f, 32 if s % 2 == 0 else 64))
+    return out
