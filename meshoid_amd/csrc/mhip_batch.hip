@@ -17,8 +17,9 @@
 // deferred-check mode (no host synchronisation inside the graph); their
 // bbox / NaN records are validated after each run.
 //
-// The contraction's field tile (fields per pass over the neighbours: 2, 4, 8
-// or 16) is chosen by timing every candidate on the first eager run.
+// The contraction's field tile (fields per thread: 2, 4, 8 or 16; the K / tile
+// tiles run as parallel threads) is chosen by timing every candidate on the
+// first eager run.
 #include <algorithm>
 #include <new>
 #include <numeric>
@@ -239,7 +240,7 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
                 float ms = 0.f;
                 for (int rep = 0; rep < 3; rep++) {
                     (void)hipEventRecord(e0, ctx->stream);
-                    rc = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, KC_CAND[c]);
+                    rc = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, KC_CAND[c], b->K);
                     (void)hipEventRecord(e1, ctx->stream);
                     if (rc) break;
                     (void)hipEventSynchronize(e1);
@@ -258,7 +259,7 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
             (void)hipEventDestroy(e1);
             if (rc) return rc;
         }
-        if ((rc = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, b->kc))) return rc;
+        if ((rc = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, b->kc, b->K))) return rc;
         b->warmed = true;
         return validate(ctx, b);
     }
@@ -269,7 +270,7 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
         rc = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "batch capture");
         if (!rc) {
             int rce = enqueue_snapshots(ctx, b, pos, f, h, rho, status);
-            if (!rce) rce = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, b->kc);
+            if (!rce) rce = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, b->kc, b->K);
             hipGraph_t g = nullptr;
             const int rc2 = check_hip(ctx, hipStreamEndCapture(ctx->stream, &g), "batch capture end");
             rc = rce ? rce : rc2;

@@ -23,9 +23,10 @@ namespace mhip {
 // ---------------------------------------------------------------------------
 
 
-// one query slot s of the list: KC fields per pass over the neighbours
+// one query slot s of the list, fields [fb, fe): KC fields per pass over the
+// neighbours
 template <int DIM, int ORDER, typename FT, int KC>
-MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s) {
+MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe) {
     constexpr int ND = NDer<DIM, ORDER>::ND, NE = NDer<DIM, ORDER>::NE;
     const uint32_t qp = a.qsorted ? a.qsorted[s] : (uint32_t)s;
     const uint32_t row = a.qrow ? a.qrow[s] : a.perm[s];
@@ -40,7 +41,7 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s) {
     const double ih = hh > 0.f ? 1.0 / (double)hh : (double)INFINITY;
     const FT *fs = (const FT *)a.fs;
     int32_t st = 0;
-    for (int f0 = 0; f0 < a.K; f0 += KC) {
+    for (int f0 = fb; f0 < fe; f0 += KC) {
         double E[NE];
         double B[KC][ND];
         FT fc[KC];
@@ -116,12 +117,14 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t s = (int64_t)b * blockDim.x + threadIdx.x;
     if (s >= a.nq) return;
-    apply_row<DIM, ORDER, FT, KC>(a, s);
+    apply_row<DIM, ORDER, FT, KC>(a, s, 0, a.K);
 }
 
 // every snapshot of a batch in one launch: blk0[i] is the first block of
-// snapshot i (blk0[nsnap] = grid size); each block finds its snapshot's
-// arguments in the device table
+// snapshot i (blk0[nsnap] = grid.x); each block finds its snapshot's
+// arguments in the device table.  grid.y splits the fields into tiles of KC:
+// the threads of a tile recompute the moment matrix, so tiles run in parallel
+// instead of as passes of one thread (their status bits are identical).
 template <int KC>
 __global__ __launch_bounds__(256) void k_apply_batch(const ApplyArgs *__restrict__ tab,
                                                      const int64_t *__restrict__ blk0, int nsnap) {
@@ -135,12 +138,14 @@ __global__ __launch_bounds__(256) void k_apply_batch(const ApplyArgs *__restrict
     const ApplyArgs a = tab[lo];
     const int64_t s = (b - blk0[lo]) * blockDim.x + threadIdx.x;
     if (s >= a.nq) return;
-    apply_row<3, 1, float, KC>(a, s);
+    const int fb = (int)blockIdx.y * KC;
+    if (fb >= a.K) return;
+    apply_row<3, 1, float, KC>(a, s, fb, min(fb + KC, a.K));
 }
 
 int apply_batch_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
-                       int64_t nblocks, int kc) {
-    const dim3 g((unsigned)nblocks), b(256);
+                       int64_t nblocks, int kc, int K) {
+    const dim3 g((unsigned)nblocks, (unsigned)((K + kc - 1) / kc)), b(256);
     switch (kc) {
         case 2: hipLaunchKernelGGL(k_apply_batch<2>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
         case 4: hipLaunchKernelGGL(k_apply_batch<4>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;

@@ -71,9 +71,10 @@ struct ApplyArgs {
 };
 
 // every snapshot of a batch in one launch (table and block prefix on the
-// device); kc = fields per pass over the neighbours (2, 4, 8 or 16)
+// device); kc = fields per thread (2, 4, 8 or 16), K / kc field tiles in
+// parallel
 int apply_batch_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
-                       int64_t nblocks, int kc);
+                       int64_t nblocks, int kc, int K);
 void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, int32_t *status,
                      int or_status, ApplyArgs *a);
 
