@@ -209,7 +209,7 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-n", type=int, default=1 << 21)
+    ap.add_argument("--cpu-n", type=int, default=1 << 24)
     ap.add_argument("--check", type=int, default=1 << 16, help="oracle sample size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stats", action="store_true", help="kNN work counters (slower)")
