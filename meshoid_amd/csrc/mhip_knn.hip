@@ -635,26 +635,40 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         // (the (k+1)-th distance is strictly larger), so "d2 <= tau" selects
         // the members.  Ambiguous rows (float32 ties at tau included) are
         // rebuilt by the float64 resolver; for them the count is only capped.
+        // Each lane scans only the recorded leaves its own final ball reaches,
+        // 32 recorded leaves at a time: a step scans one leaf per lane (its
+        // particles loaded by the lane itself), so the wave pays for the
+        // busiest lane, not for every leaf some lane needs.
         int cnt = 0;
         uint32_t nscan = 0;
         const float tb = tau * PRUNE_REL;
-        for (uint32_t r = 0; r < nwl; r++) {
-            const uint32_t li = uniform(s_wl[w][r]);
-            if (!__any(box_gap2_u<PER>(t, qb, first_leaf + li) <= tb)) continue;
-            nscan++;
-            load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, li, cx, cy, cz, clx, cly, clz);
+        for (uint32_t r0 = 0; r0 < nwl; r0 += 32) {
+            const uint32_t nr = min(32u, nwl - r0);
+            uint32_t need = 0;
+            for (uint32_t r = 0; r < nr; r++) {
+                const uint32_t li = uniform(s_wl[w][r0 + r]);
+                if (box_gap2_u<PER>(t, qb, first_leaf + li) <= tb) need |= 1u << r;
+            }
+            while (__any(need != 0u)) {
+                const bool act = need != 0u;
+                const uint32_t r = act ? (uint32_t)__builtin_ctz(need) : 0u;
+                need &= need - 1u;
+                const uint32_t li = s_wl[w][r0 + r];
+                nscan++;
+                load_leaf<P64>(t, li, cx, cy, cz, clx, cly, clz);
 #pragma unroll
-            for (int i = 0; i < LEAF; i++) {
-                const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
-                const bool take = (d2 <= tau) & (cnt < a.k);
-                if (a.nl_d2 && take && live) a.nl_d2[(int64_t)cnt * a.nq + slot] = d2;
-                if (STAGE) {
-                    // row cnt <= k < LROWS; the row is kept only on a hit
-                    s_buf[w][cnt][lane] = li * LEAF + i;
-                } else if (take && live) {
-                    a.nl_idx[(int64_t)cnt * a.nq + slot] = li * LEAF + i;
+                for (int i = 0; i < LEAF; i++) {
+                    const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
+                    const bool take = act & (d2 <= tau) & (cnt < a.k);
+                    if (a.nl_d2 && take && live) a.nl_d2[(int64_t)cnt * a.nq + slot] = d2;
+                    if (STAGE) {
+                        // row cnt <= k < LROWS; the row is kept only on a hit
+                        s_buf[w][cnt][lane] = li * LEAF + i;
+                    } else if (take && live) {
+                        a.nl_idx[(int64_t)cnt * a.nq + slot] = li * LEAF + i;
+                    }
+                    cnt += take;
                 }
-                cnt += take;
             }
         }
         if (a.stats && lane == 0) atomicAdd(a.stats + STAT_P2_NODES, (unsigned long long)nscan);
