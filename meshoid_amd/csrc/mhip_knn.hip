@@ -636,23 +636,31 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         // the members.  Ambiguous rows (float32 ties at tau included) are
         // rebuilt by the float64 resolver; for them the count is only capped.
         // Each lane scans only the recorded leaves its own final ball reaches,
-        // 32 recorded leaves at a time: a step scans one leaf per lane (its
+        // 64 recorded leaves at a time: a step scans one leaf per lane (its
         // particles loaded by the lane itself), so the wave pays for the
         // busiest lane, not for every leaf some lane needs.
         int cnt = 0;
         uint32_t nscan = 0;
         const float tb = tau * PRUNE_REL;
-        for (uint32_t r0 = 0; r0 < nwl; r0 += 32) {
-            const uint32_t nr = min(32u, nwl - r0);
-            uint32_t need = 0;
+        for (uint32_t r0 = 0; r0 < nwl; r0 += 64) {
+            const uint32_t nr = min(64u, nwl - r0);
+            uint32_t nlo = 0, nhi = 0;
             for (uint32_t r = 0; r < nr; r++) {
                 const uint32_t li = uniform(s_wl[w][r0 + r]);
-                if (box_gap2_u<PER>(t, qb, first_leaf + li) <= tb) need |= 1u << r;
+                if (box_gap2_u<PER>(t, qb, first_leaf + li) <= tb) {
+                    if (r < 32)
+                        nlo |= 1u << r;
+                    else
+                        nhi |= 1u << (r - 32);
+                }
             }
-            while (__any(need != 0u)) {
-                const bool act = need != 0u;
-                const uint32_t r = act ? (uint32_t)__builtin_ctz(need) : 0u;
-                need &= need - 1u;
+            while (__any((nlo | nhi) != 0u)) {
+                const bool act = (nlo | nhi) != 0u;
+                const uint32_t r = nlo ? (uint32_t)__builtin_ctz(nlo) : (nhi ? 32u + (uint32_t)__builtin_ctz(nhi) : 0u);
+                if (nlo)
+                    nlo &= nlo - 1u;
+                else
+                    nhi &= nhi - 1u;
                 const uint32_t li = s_wl[w][r0 + r];
                 nscan++;
                 load_leaf<P64>(t, li, cx, cy, cz, clx, cly, clz);
