@@ -120,15 +120,21 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
     apply_row<DIM, ORDER, FT, KC>(a, s, 0, a.K);
 }
 
-// every snapshot of a batch in one launch: blk0[i] is the first block of
-// snapshot i (blk0[nsnap] = grid.x); each block finds its snapshot's
-// arguments in the device table.  grid.y splits the fields into tiles of KC:
-// the threads of a tile recompute the moment matrix, so tiles run in parallel
-// instead of as passes of one thread (their status bits are identical).
+// every snapshot of a batch in one launch: blk0[i] is the first query block
+// of snapshot i (blk0[nsnap] = query blocks); each block finds its
+// snapshot's arguments in the device table.  The fields are split into ntile
+// tiles of KC: the threads of a tile recompute the moment matrix, so tiles
+// run in parallel instead of as passes of one thread (their status bits are
+// identical).
 template <int KC>
 __global__ __launch_bounds__(256) void k_apply_batch(const ApplyArgs *__restrict__ tab,
-                                                     const int64_t *__restrict__ blk0, int nsnap) {
-    const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+                                                     const int64_t *__restrict__ blk0, int nsnap, int ntile) {
+    // the tiles of one block of queries are consecutive logical blocks, so
+    // they run together on one XCD and share its L2 for the neighbour
+    // indices, positions and nearby field values
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t b = lb / (uint32_t)ntile;
+    const int tile = (int)(lb % (uint32_t)ntile);
     int lo = 0, hi = nsnap;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -138,19 +144,21 @@ __global__ __launch_bounds__(256) void k_apply_batch(const ApplyArgs *__restrict
     const ApplyArgs a = tab[lo];
     const int64_t s = (b - blk0[lo]) * blockDim.x + threadIdx.x;
     if (s >= a.nq) return;
-    const int fb = (int)blockIdx.y * KC;
+    const int fb = tile * KC;
     if (fb >= a.K) return;
     apply_row<3, 1, float, KC>(a, s, fb, min(fb + KC, a.K));
 }
 
 int apply_batch_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
                        int64_t nblocks, int kc, int K) {
-    const dim3 g((unsigned)nblocks, (unsigned)((K + kc - 1) / kc)), b(256);
+    const int nt = (K + kc - 1) / kc;
+    if (nblocks * nt >= ((int64_t)1 << 31)) return fail(ctx, -4, "batch too large for one launch");
+    const dim3 g((unsigned)(nblocks * nt)), b(256);
     switch (kc) {
-        case 2: hipLaunchKernelGGL(k_apply_batch<2>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
-        case 4: hipLaunchKernelGGL(k_apply_batch<4>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
-        case 8: hipLaunchKernelGGL(k_apply_batch<8>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
-        case 16: hipLaunchKernelGGL(k_apply_batch<16>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        case 2: hipLaunchKernelGGL(k_apply_batch<2>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap, nt); break;
+        case 4: hipLaunchKernelGGL(k_apply_batch<4>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap, nt); break;
+        case 8: hipLaunchKernelGGL(k_apply_batch<8>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap, nt); break;
+        case 16: hipLaunchKernelGGL(k_apply_batch<16>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap, nt); break;
         default: return fail(ctx, -1, "unsupported field tile");
     }
     return check_hip(ctx, hipGetLastError(), "batched apply launch");
