@@ -206,15 +206,18 @@ int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, in
  * spread over nbranch streams; with use_graph the whole batch (fork/join of
  * the branches + one contraction launch over all snapshots) is captured once
  * into a HIP graph and replayed (recaptured when the buffers change).  The
- * first run is eager and times the contraction's field tile (2/4/8/16 fields
- * per pass); mhip_batch_info reports the choice.  Non-periodic sets only.
+ * first run is eager and times the contraction's candidates (K >= 8: a group of
+ * 16/32/64 threads per query over row-major fields, wide = 1; fewer fields:
+ * 2/4/8/16 fields per thread); mhip_batch_info reports them and the choice.
+ * Non-periodic sets only.
  */
 int mhip_batch_create(mhip_ctx *ctx, int nsnap, const int64_t *n, const int *k, int K, int nbranch,
                       mhip_batch **out);
 int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const float *const *f,
                    float *const *h, float *const *rho, float *const *grad, int32_t *const *status,
                    int use_graph);
-int mhip_batch_info(const mhip_batch *b, int *field_tile, float *tune_ms4, int *nbranch);
+int mhip_batch_info(const mhip_batch *b, int *wide, int *choice, int *candidates4, float *tune_ms4,
+                    int *nbranch);
 int mhip_batch_destroy(mhip_batch *b);
 
 /*

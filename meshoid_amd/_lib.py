@@ -77,7 +77,7 @@ _SIGS = {
     "mhip_integrate": ([_p, _p, _p, _i, _i64, _i, _p], _i),
     "mhip_batch_create": ([_p, _i, _p, _p, _i, _i, _pp], _i),
     "mhip_batch_run": ([_p, _p, _p, _p, _p, _p, _p, _p, _i], _i),
-    "mhip_batch_info": ([_p, _p, _p, _p], _i),
+    "mhip_batch_info": ([_p, _p, _p, _p, _p, _p], _i),
     "mhip_batch_destroy": ([_p], _i),
 }
 for _name, (_args, _res) in _SIGS.items():

@@ -88,12 +88,16 @@ class SnapshotBatch:
         return o
 
     def info(self):
-        kc = ctypes.c_int()
-        nb = ctypes.c_int()
+        """The contraction kernel chosen by the first run's timing: kernel
+        "wide" (threads per query) or "tile" (fields per thread), the chosen
+        value and the measured ms of every candidate."""
+        wide, choice, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        cand = (ctypes.c_int * 4)()
         ms = (ctypes.c_float * 4)()
-        raise_for(lib.mhip_batch_info(self.h, ctypes.byref(kc), ms, ctypes.byref(nb)))
-        return {"field_tile": kc.value, "branches": nb.value,
-                "tile_ms": {t: round(float(ms[i]), 4) for i, t in enumerate((2, 4, 8, 16))}}
+        raise_for(lib.mhip_batch_info(self.h, ctypes.byref(wide), ctypes.byref(choice), cand, ms,
+                                      ctypes.byref(nb)))
+        return {"kernel": "wide" if wide.value else "tile", "choice": choice.value, "branches": nb.value,
+                "candidates_ms": {int(cand[i]): round(float(ms[i]), 4) for i in range(4) if cand[i]}}
 
     def __del__(self):
         h = getattr(self, "h", None)

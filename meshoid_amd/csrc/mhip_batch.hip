@@ -17,9 +17,10 @@
 // deferred-check mode (no host synchronisation inside the graph); their
 // bbox / NaN records are validated after each run.
 //
-// The contraction's field tile (fields per thread: 2, 4, 8 or 16; the K / tile
-// tiles run as parallel threads) is chosen by timing every candidate on the
-// first eager run.
+// The contraction kernel: with K >= 8 fields, k_apply_wide (a thread group
+// per query, row-major fields; group width 16, 32 or 64 threads); with fewer,
+// the tile kernel (2, 4, 8 or 16 fields per thread).  The variant is chosen by
+// timing every candidate on the first eager run.
 #include <algorithm>
 #include <new>
 #include <numeric>
@@ -42,10 +43,11 @@ struct mhip_batch {
     float *fsorted = nullptr;           // per snapshot [K][npad] sorted-space fields
     std::vector<int64_t> foff;          // offsets into fsorted (floats)
     mhip::ApplyArgs *tab_d = nullptr;   // [nsnap]
-    int64_t *blk0_d = nullptr;          // [nsnap + 1]
-    int64_t nblocks = 0;
-    int kc = 0;                         // chosen field tile
-    float tune_ms[4] = {0, 0, 0, 0};    // per candidate 2, 4, 8, 16
+    int wide = 0;                       // contraction kernel family (1: k_apply_wide, row-major fields)
+    int64_t *blk0_d = nullptr;          // per candidate [nsnap + 1] block prefixes
+    int64_t nblocks[4] = {0, 0, 0, 0};  // per candidate
+    int choice = -1;                    // chosen candidate
+    float tune_ms[4] = {0, 0, 0, 0};    // per candidate
     bool warmed = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -54,7 +56,21 @@ struct mhip_batch {
 
 namespace mhip {
 
-static const int KC_CAND[4] = {2, 4, 8, 16};
+// contraction candidates: fields per thread for the tile kernel (256
+// queries per block), threads per query for the wide kernel
+struct Cand {
+    int param, qpb;
+};
+static const Cand TILE_CAND[4] = {{2, 256}, {4, 256}, {8, 256}, {16, 256}};
+static const Cand WIDE_CAND[4] = {{16, 16}, {32, 8}, {64, 4}, {0, 0}};
+static int ncand(const mhip_batch *b) { return b->wide ? 3 : 4; }
+static Cand cand(const mhip_batch *b, int c) { return b->wide ? WIDE_CAND[c] : TILE_CAND[c]; }
+
+static int launch_contraction(mhip_ctx *ctx, mhip_batch *b, int c) {
+    const int64_t *blk0 = b->blk0_d + (int64_t)c * (b->nsnap + 1);
+    if (b->wide) return apply_wide_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cand(b, c).param);
+    return apply_batch_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cand(b, c).param, b->K);
+}
 
 static int64_t npad_of(int64_t n) { return (n + LEAF - 1) / LEAF * LEAF; }
 
@@ -68,8 +84,12 @@ static int enqueue_snapshot(mhip_batch *b, int s, const float *pos, const float 
                  0, b->nl[s]);
     if (rc) return rc;
     const int64_t np = b->ix[s]->npad;
-    hipLaunchKernelGGL(k_permute_fields<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, f,
-                       b->n[s], b->K, b->ix[s]->perm, b->n[s], np, b->fsorted + b->foff[s]);
+    if (b->wide)
+        hipLaunchKernelGGL(k_permute_fields_rows<float>, dim3((unsigned)((np * b->K + 255) / 256)), dim3(256), 0,
+                           c->stream, f, b->n[s], b->K, b->ix[s]->perm, b->n[s], np, b->fsorted + b->foff[s]);
+    else
+        hipLaunchKernelGGL(k_permute_fields<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, f,
+                           b->n[s], b->K, b->ix[s]->perm, b->n[s], np, b->fsorted + b->foff[s]);
     return check_hip(c, hipGetLastError(), "batch field permutation");
 }
 
@@ -96,12 +116,16 @@ static int enqueue_snapshots(mhip_ctx *ctx, mhip_batch *b, const float *const *p
 
 static int upload_table(mhip_ctx *ctx, mhip_batch *b, float *const *grad, int32_t *const *status) {
     std::vector<ApplyArgs> tab(b->nsnap);
-    std::vector<int64_t> blk0(b->nsnap + 1, 0);
-    for (int s = 0; s < b->nsnap; s++) {
+    const int nc = ncand(b);
+    std::vector<int64_t> blk0((size_t)nc * (b->nsnap + 1), 0);
+    for (int s = 0; s < b->nsnap; s++)
         apply_args_fill(b->nl[s], b->fsorted + b->foff[s], b->K, grad[s], status[s], 1, &tab[s]);
-        blk0[s + 1] = blk0[s] + (b->n[s] + 255) / 256;
+    for (int c = 0; c < nc; c++) {
+        int64_t *bc = blk0.data() + (size_t)c * (b->nsnap + 1);
+        const int64_t qpb = cand(b, c).qpb;
+        for (int s = 0; s < b->nsnap; s++) bc[s + 1] = bc[s] + (b->n[s] + qpb - 1) / qpb;
+        b->nblocks[c] = bc[b->nsnap];
     }
-    b->nblocks = blk0[b->nsnap];
     int rc = check_hip(ctx, hipMemcpy(b->tab_d, tab.data(), tab.size() * sizeof(ApplyArgs), hipMemcpyHostToDevice),
                        "batch table upload");
     if (rc) return rc;
@@ -153,6 +177,7 @@ int mhip_batch_create(mhip_ctx *ctx, int nsnap, const int64_t *n, const int *k, 
     b->nsnap = nsnap;
     b->K = K;
     b->nbranch = nbranch;
+    b->wide = K >= 8;  // many fields: one query per thread group, coalesced field rows
     b->n.assign(n, n + nsnap);
     b->k.assign(k, k + nsnap);
     // largest snapshots first, each to the least loaded branch
@@ -191,7 +216,7 @@ int mhip_batch_create(mhip_ctx *ctx, int nsnap, const int64_t *n, const int *k, 
         return bail(rc);
     if ((rc = check_hip(ctx, hipMalloc((void **)&b->tab_d, (size_t)nsnap * sizeof(ApplyArgs)), "hipMalloc (batch)")))
         return bail(rc);
-    if ((rc = check_hip(ctx, hipMalloc((void **)&b->blk0_d, (size_t)(nsnap + 1) * sizeof(int64_t)), "hipMalloc (batch)")))
+    if ((rc = check_hip(ctx, hipMalloc((void **)&b->blk0_d, (size_t)4 * (nsnap + 1) * sizeof(int64_t)), "hipMalloc (batch)")))
         return bail(rc);
     for (int s = 0; s < nsnap; s++) {
         mhip_index *x = nullptr;
@@ -230,17 +255,17 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
         // exist once the snapshots are enqueued.
         if ((rc = enqueue_snapshots(ctx, b, pos, f, h, rho, status))) return rc;
         if ((rc = upload_table(ctx, b, grad, status))) return rc;
-        if (!b->kc) {
+        if (b->choice < 0) {
             if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch warm-up"))) return rc;
             hipEvent_t e0, e1;
             (void)hipEventCreate(&e0);
             (void)hipEventCreate(&e1);
             float best = 0.f;
-            for (int c = 0; c < 4; c++) {
+            for (int c = 0; c < ncand(b); c++) {
                 float ms = 0.f;
                 for (int rep = 0; rep < 3; rep++) {
                     (void)hipEventRecord(e0, ctx->stream);
-                    rc = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, KC_CAND[c], b->K);
+                    rc = launch_contraction(ctx, b, c);
                     (void)hipEventRecord(e1, ctx->stream);
                     if (rc) break;
                     (void)hipEventSynchronize(e1);
@@ -250,8 +275,8 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
                 }
                 if (rc) break;
                 b->tune_ms[c] = ms;
-                if (!b->kc || ms < best) {
-                    b->kc = KC_CAND[c];
+                if (b->choice < 0 || ms < best) {
+                    b->choice = c;
                     best = ms;
                 }
             }
@@ -259,7 +284,7 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
             (void)hipEventDestroy(e1);
             if (rc) return rc;
         }
-        if ((rc = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, b->kc, b->K))) return rc;
+        if ((rc = launch_contraction(ctx, b, b->choice))) return rc;
         b->warmed = true;
         return validate(ctx, b);
     }
@@ -270,7 +295,7 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
         rc = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "batch capture");
         if (!rc) {
             int rce = enqueue_snapshots(ctx, b, pos, f, h, rho, status);
-            if (!rce) rce = apply_batch_launch(ctx, b->tab_d, b->blk0_d, b->nsnap, b->nblocks, b->kc, b->K);
+            if (!rce) rce = launch_contraction(ctx, b, b->choice);
             hipGraph_t g = nullptr;
             const int rc2 = check_hip(ctx, hipStreamEndCapture(ctx->stream, &g), "batch capture end");
             rc = rce ? rce : rc2;
@@ -292,11 +317,15 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
     return validate(ctx, b);
 }
 
-int mhip_batch_info(const mhip_batch *b, int *kc, float *tune_ms, int *nbranch) {
+int mhip_batch_info(const mhip_batch *b, int *wide, int *choice, int *cand4, float *tune_ms4, int *nbranch) {
     if (!b) return MHIP_ERR_VALUE;
-    if (kc) *kc = b->kc;
-    if (tune_ms)
-        for (int c = 0; c < 4; c++) tune_ms[c] = b->tune_ms[c];
+    if (wide) *wide = b->wide;
+    if (choice) *choice = b->choice >= 0 ? cand(b, b->choice).param : 0;
+    for (int c = 0; c < 4; c++) {
+        const bool in = c < ncand(b);
+        if (cand4) cand4[c] = in ? cand(b, c).param : 0;
+        if (tune_ms4) tune_ms4[c] = in ? b->tune_ms[c] : 0.f;
+    }
     if (nbranch) *nbranch = b->nbranch;
     return MHIP_OK;
 }

@@ -149,6 +149,110 @@ __global__ __launch_bounds__(256) void k_apply_batch(const ApplyArgs *__restrict
     apply_row<3, 1, float, KC>(a, s, fb, min(fb + KC, a.K));
 }
 
+// Many fields at once (batched form, order 1, 3D, float32 positions and
+// row-major fields [npad][K]): a block holds G = 256 / KW queries with KW
+// threads each.  Phase 1 spreads a query's neighbours over its KW threads:
+// each computes w_j a_j, the moment matrix is summed over the group with
+// shuffles, every thread factorises it, and the owner of neighbour j stores
+// g_j = E^-1 w_j a_j in LDS.  Phase 2 gives each thread fields f, f + KW, ...:
+// out[f] = sum_j g_j (f_j - f_i) in float64, where the group's KW threads read
+// one neighbour's KW consecutive field values (one cache line) together.
+// Same operator as apply_row, summed in another order.
+constexpr int WIDE_KMAX = 128;
+
+template <int KW>
+__global__ __launch_bounds__(256) void k_apply_wide(const ApplyArgs *__restrict__ tab,
+                                                    const int64_t *__restrict__ blk0, int nsnap) {
+    constexpr int G = 256 / KW;
+    __shared__ double s_g[G][WIDE_KMAX][3];
+    __shared__ uint32_t s_p[G][WIDE_KMAX];
+    const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    int lo = 0, hi = nsnap;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (blk0[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const ApplyArgs a = tab[lo];
+    const int g = threadIdx.x / KW, f = threadIdx.x % KW;
+    const int64_t s = (b - blk0[lo]) * G + g;
+    const bool live = s < a.nq;
+    const int64_t sc = live ? s : a.nq - 1;  // padding queries repeat the last one
+    const uint32_t qp = a.qsorted ? a.qsorted[sc] : (uint32_t)sc;
+    const uint32_t row = a.qrow ? a.qrow[sc] : a.perm[sc];
+    const double qx = (double)a.sx[qp], qy = (double)a.sy[qp], qz = (double)a.sz[qp];
+    const float hh = a.h[sc];
+    const double ih = hh > 0.f ? 1.0 / (double)hh : (double)INFINITY;
+    const int k = a.k;
+    // phase 1: w_j a_j and the moment matrix
+    double E[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = f; j < k; j += KW) {
+        const uint32_t p = a.nl_idx[(int64_t)j * a.nq + sc];
+        const double d[3] = {(double)a.sx[p] - qx, (double)a.sy[p] - qy, (double)a.sz[p] - qz};
+        const double w = kernel_w<double>(sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
+        double wa[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) wa[i] = w * d[i];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+#pragma unroll
+            for (int jj = 0; jj <= i; jj++) E[tri(i, jj)] = fma(wa[i], d[jj], E[tri(i, jj)]);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) s_g[g][j][i] = wa[i];
+        s_p[g][j] = p;
+    }
+#pragma unroll
+    for (int o = KW / 2; o > 0; o >>= 1) {
+#pragma unroll
+        for (int e = 0; e < 6; e++) E[e] += __shfl_xor(E[e], o);
+    }
+    int32_t st = 0;
+    if (zero_diagonal<3>(E)) st |= ST_SINGULAR;
+    const bool ok = ldlt<3>(E);
+    if (!ok) st |= ST_RANKDEF;
+    __syncthreads();
+    for (int j = f; j < k; j += KW) {
+        double v[3] = {s_g[g][j][0], s_g[g][j][1], s_g[g][j][2]};
+        ldlt_solve<3>(E, v);
+#pragma unroll
+        for (int i = 0; i < 3; i++) s_g[g][j][i] = v[i];
+    }
+    __syncthreads();
+    // phase 2: fields f, f + KW, ...
+    const float *fs = (const float *)a.fs;
+    for (int c = f; c < a.K; c += KW) {
+        const double fi = (double)fs[(int64_t)qp * a.K + c];
+        double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+        for (int j = 0; j < k; j++) {
+            const double df = (double)fs[(int64_t)s_p[g][j] * a.K + c] - fi;
+            o0 = fma(s_g[g][j][0], df, o0);
+            o1 = fma(s_g[g][j][1], df, o1);
+            o2 = fma(s_g[g][j][2], df, o2);
+        }
+        if (!ok) o0 = o1 = o2 = (double)NAN;
+        if (live) {
+            float *o = a.out1 + ((int64_t)row * a.K + c) * 3;
+            o[0] = (float)o0;
+            o[1] = (float)o1;
+            o[2] = (float)o2;
+        }
+    }
+    if (live && f == 0 && a.status) a.status[row] = a.or_status ? (a.status[row] | st) : st;
+}
+
+int apply_wide_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
+                      int64_t nblocks, int kw) {
+    const dim3 g((unsigned)nblocks), b(256);
+    switch (kw) {
+        case 16: hipLaunchKernelGGL(k_apply_wide<16>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        case 32: hipLaunchKernelGGL(k_apply_wide<32>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        case 64: hipLaunchKernelGGL(k_apply_wide<64>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        default: return fail(ctx, -1, "unsupported field group");
+    }
+    return check_hip(ctx, hipGetLastError(), "wide apply launch");
+}
+
 int apply_batch_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
                        int64_t nblocks, int kc, int K) {
     const int nt = (K + kc - 1) / kc;

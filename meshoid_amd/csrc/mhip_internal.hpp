@@ -77,6 +77,25 @@ int apply_batch_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk
                        int64_t nblocks, int kc, int K);
 void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, int32_t *status,
                      int or_status, ApplyArgs *a);
+// many fields at once (order 1, 3D, float32 positions, row-major fields
+// [npad][K], des_ngb <= 128): kw threads per query (16, 32 or 64), a block
+// holds 256 / kw queries; blk0 counts query blocks of that size
+int apply_wide_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
+                      int64_t nblocks, int kw);
+
+// field rows (original order, K components each) -> sorted-space rows
+// [npad][K]; rows beyond nf_rows and the padding read NaN
+template <typename FT>
+__global__ __launch_bounds__(256) void k_permute_fields_rows(const FT *__restrict__ f, int64_t nf_rows, int K,
+                                                            const uint32_t *__restrict__ perm, int64_t n,
+                                                            int64_t npad, FT *__restrict__ fs) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= npad * K) return;
+    const int64_t p = e / K;
+    const int c = (int)(e - p * K);
+    const uint32_t o = p < n ? perm[p] : 0xffffffffu;
+    fs[e] = (o < nf_rows) ? f[(int64_t)o * K + c] : (FT)NAN;
+}
 
 // kernel-weighted neighbour statistics (mhip_kops.hip)
 int kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w);
