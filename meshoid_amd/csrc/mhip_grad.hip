@@ -24,9 +24,11 @@ namespace mhip {
 
 
 // one query slot s of the list, fields [fb, fe): KC fields per pass over the
-// neighbours
-template <int DIM, int ORDER, typename FT, int KC>
-MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe) {
+// neighbours.  With GW > 1 a group of GW consecutive lanes serves the query:
+// lane gl takes neighbours gl, gl + GW, ..., the moment matrix and right-hand
+// sides are summed over the group with shuffles, and lane 0 writes.
+template <int DIM, int ORDER, typename FT, int KC, int GW = 1>
+MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 0) {
     constexpr int ND = NDer<DIM, ORDER>::ND, NE = NDer<DIM, ORDER>::NE;
     const uint32_t qp = a.qsorted ? a.qsorted[s] : (uint32_t)s;
     const uint32_t row = a.qrow ? a.qrow[s] : a.perm[s];
@@ -53,7 +55,7 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe) {
             for (int i = 0; i < ND; i++) B[f][i] = 0.0;
             fc[f] = f0 + f < a.K ? fs[(int64_t)(f0 + f) * a.npad + qp] : FT(0);
         }
-        for (int j = 0; j < a.k; j++) {
+        for (int j = gl; j < a.k; j += GW) {
             const uint32_t p = a.nl_idx[(int64_t)j * a.nq + s];
             double d[3] = {(double)a.sx[p] - qx, (double)a.sy[p] - qy, (double)a.sz[p] - qz};
             if (a.p64) {
@@ -86,12 +88,24 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe) {
                 }
             }
         }
+        if (GW > 1) {
+#pragma unroll
+            for (int o = GW / 2; o > 0; o >>= 1) {
+#pragma unroll
+                for (int i = 0; i < NE; i++) E[i] += __shfl_xor(E[i], o);
+#pragma unroll
+                for (int f = 0; f < KC; f++) {
+#pragma unroll
+                    for (int i = 0; i < ND; i++) B[f][i] += __shfl_xor(B[f][i], o);
+                }
+            }
+        }
         if (zero_diagonal<ND>(E)) st |= ST_SINGULAR;
         const bool ok = ldlt<ND>(E);
         if (!ok) st |= ST_RANKDEF;
 #pragma unroll
         for (int f = 0; f < KC; f++) {
-            if (f0 + f >= a.K) continue;
+            if (f0 + f >= a.K || gl != 0) continue;
             ldlt_solve<ND>(E, B[f]);
             if (!ok) {
 #pragma unroll
@@ -109,15 +123,23 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe) {
             }
         }
     }
-    if (a.status) a.status[row] = a.or_status ? (a.status[row] | st) : st;
+    if (a.status && gl == 0) a.status[row] = a.or_status ? (a.status[row] | st) : st;
 }
+
+// lanes per query: 4 for first-order fits (more neighbour gathers in flight;
+// measured 2.79 vs 3.23 ms at 2^24, k = 32), 1 for the 9x9 second-order fits,
+// whose 63 float64 sums would cost more to combine than they gain (9.5 vs 7.0 ms)
+template <int ORDER>
+constexpr int apply_gw() { return ORDER == 1 ? 4 : 1; }
 
 template <int DIM, int ORDER, typename FT, int KC>
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
+    constexpr int GW = apply_gw<ORDER>();
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t s = (int64_t)b * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)b * blockDim.x + threadIdx.x;
+    const int64_t s = t / GW;
     if (s >= a.nq) return;
-    apply_row<DIM, ORDER, FT, KC>(a, s, 0, a.K);
+    apply_row<DIM, ORDER, FT, KC, GW>(a, s, 0, a.K, (int)(t % GW));
 }
 
 // every snapshot of a batch in one launch: blk0[i] is the first query block
@@ -300,7 +322,10 @@ void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, i
 
 template <int DIM, int ORDER, typename FT>
 static void launch_apply(const ApplyArgs &a, dim3 g, hipStream_t s) {
-    if (ORDER == 1)
+    // one field: no unused right-hand sides in the registers and shuffles
+    if (a.K == 1)
+        hipLaunchKernelGGL((k_apply<DIM, ORDER, FT, 1>), g, dim3(256), 0, s, a);
+    else if (ORDER == 1)
         hipLaunchKernelGGL((k_apply<DIM, ORDER, FT, 4>), g, dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((k_apply<DIM, ORDER, FT, 2>), g, dim3(256), 0, s, a);
@@ -365,7 +390,8 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     a.or_status = 0;
     prp.end();
     ProfRegion pra(ctx, order == 1 ? "apply_D1" : "apply_D2");
-    const dim3 g((unsigned)((nl->nq + 255) / 256));
+    const int gw = order == 1 ? apply_gw<1>() : apply_gw<2>();
+    const dim3 g((unsigned)((nl->nq * gw + 255) / 256));
     if (f_f64)
         dispatch_apply<double>(a, ix->dim, order, g, ctx->stream);
     else

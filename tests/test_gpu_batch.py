@@ -1,6 +1,8 @@
 """Batched form: every snapshot of a SnapshotBatch equals the single-set path
-(same kernels, so bit for bit), eager and through graph replays, and the
-single-set path itself matches the FP64 oracle (tests/test_gpu_ops.py)."""
+(the same index and kNN kernels bit for bit; the derivative contraction up to
+float64 summation order), eager and through graph replays (bit for bit
+against the eager batch), and the single-set path itself matches the FP64
+oracle (tests/test_gpu_ops.py)."""
 import sys
 
 import numpy as np
@@ -45,14 +47,46 @@ def same(a, b):
     return True
 
 
+def moment_cond(pos, ngb, h):
+    """cond of each row's weighted 3x3 moment matrix sum_j W(r_j/h) d_j d_j^T
+    (derivatives.py:69-78), from the exported neighbour lists."""
+    from oracle import oracle as orc
+
+    p = pos.astype(np.float64)
+    d = p[ngb] - p[:, None, :]
+    w = orc.kernel(np.linalg.norm(d, axis=2) / h.astype(np.float64)[:, None])
+    E = np.einsum("ij,ija,ijb->iab", w, d, d)
+    with np.errstate(all="ignore"):
+        c = np.linalg.cond(E)
+    return np.where(np.isfinite(c), c, np.inf)
+
+
+def close(g_batch, g_single, s, cond):
+    """Same operator, float64 sums in another order: rows agree to
+    1e-5 + 1e-13 cond(E) relative (per field: the solve's sensitivity to the
+    summation order grows with the conditioning), NaN rows (rank-deficient)
+    in the same places."""
+    g1 = g_batch.cpu().numpy().astype(np.float64)
+    g2 = g_single.cpu().numpy().astype(np.float64)
+    ok = np.isfinite(g2).all(axis=(1, 2))
+    assert (np.isfinite(g1).all(axis=(1, 2)) == ok).all(), s
+    num = np.linalg.norm((g1 - g2)[ok], axis=2)
+    den = np.maximum(np.linalg.norm(g2[ok], axis=2), 1e-30)
+    tol = 1e-5 + 1e-13 * cond[ok][:, None]
+    say("snapshot", s, "max rel", (num / den).max(), "worst ratio to tolerance", (num / den / tol).max())
+    assert (num / den <= tol).all(), s
+
+
 def _single(pos, f, k):
     from meshoid_amd import engine
 
     ix = engine.SpatialIndex()
     ix.build(pos)
-    out = engine.knn_hsml(ix, k, keep_dist=False)
+    out = engine.knn_hsml(ix, k, keep_dist=True)
     d1, _, st = engine.apply_D(out["nlist"], f)
-    return out["h"], out["rho"], d1, out["status"] | st
+    ngb, _ = out["nlist"].export()
+    cond = moment_cond(pos.cpu().numpy(), ngb.cpu().numpy(), out["h"].cpu().numpy())
+    return out["h"], out["rho"], d1, out["status"] | st, cond
 
 
 def test_batch_matches_single_path():
@@ -70,11 +104,11 @@ def test_batch_matches_single_path():
     say("batch info", info)
     assert info["choice"] in info["candidates_ms"]
     for s in range(len(sizes)):
-        h, rho, d1, st = _single(pos[s], fld[s], ks[s])
+        h, rho, d1, st, cond = _single(pos[s], fld[s], ks[s])
         assert same(o["h"][s], h), s
         assert same(o["rho"][s], rho), s
-        assert same(o["grad"][s], d1), s
         assert same(o["status"][s], st), s
+        close(o["grad"][s], d1, s, cond)
 
 
 def test_batch_wide_contraction():
@@ -94,16 +128,9 @@ def test_batch_wide_contraction():
     say("batch info", info)
     assert info["kernel"] == "wide" and info["choice"] in (16, 32, 64)
     for s in range(len(sizes)):
-        h, rho, d1, st = _single(pos[s], fld[s], ks[s])
+        h, rho, d1, st, cond = _single(pos[s], fld[s], ks[s])
         assert same(o["h"][s], h) and same(o["rho"][s], rho) and same(o["status"][s], st), s
-        g1 = o["grad"][s].cpu().numpy().astype(np.float64)
-        g2 = d1.cpu().numpy().astype(np.float64)
-        ok = np.isfinite(g2).all(axis=(1, 2))
-        assert (np.isfinite(g1).all(axis=(1, 2)) == ok).all()
-        num = np.linalg.norm((g1 - g2)[ok], axis=2)
-        den = np.maximum(np.linalg.norm(g2[ok], axis=2), 1e-30)
-        say("snapshot", s, "max rel", (num / den).max())
-        assert (num / den).max() <= 1e-5, s
+        close(o["grad"][s], d1, s, cond)
 
 
 def test_batch_graph_replays():
