@@ -82,6 +82,10 @@ int mhip_index_build(mhip_ctx *ctx, mhip_index *ix, const void *pos, int pos_f64
 int mhip_index_destroy(mhip_index *ix);
 /* host-side copy of n, dim, boxsize and the bounding box (lo xyz, hi xyz) */
 int mhip_index_info(const mhip_index *ix, int64_t *n, int *dim, double *boxsize, float *bbox6);
+/* the index's particle order: perm_out uint32[n] (device), the original row
+ * of each sorted position (Morton order of the positions in the cubic frame
+ * of the bounding box or periodic box, 21 bits per axis, stable) */
+int mhip_index_order(mhip_ctx *ctx, const mhip_index *ix, uint32_t *perm_out);
 
 /*
  * Exact kNN (k = des_ngb, self included) + smoothing length + density for the

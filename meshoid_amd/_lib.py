@@ -29,6 +29,7 @@ EXPORTED = [
     "mhip_seed_predict", "mhip_mlp_train_step", "mhip_mlp_pack", "mhip_mlp_blob_bytes",
     "mhip_kernel_weights", "mhip_kernel_average", "mhip_kernel_variance", "mhip_integrate",
     "mhip_batch_create", "mhip_batch_run", "mhip_batch_info", "mhip_batch_destroy",
+    "mhip_index_order",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -79,6 +80,7 @@ _SIGS = {
     "mhip_batch_run": ([_p, _p, _p, _p, _p, _p, _p, _p, _i], _i),
     "mhip_batch_info": ([_p, _p, _p, _p, _p, _p], _i),
     "mhip_batch_destroy": ([_p], _i),
+    "mhip_index_order": ([_p, _p, _p], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     _fn = getattr(lib, _name)

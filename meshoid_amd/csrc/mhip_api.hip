@@ -195,6 +195,16 @@ int mhip_index_info(const mhip_index *ix, int64_t *n, int *dim, double *boxsize,
     return MHIP_OK;
 }
 
+int mhip_index_order(mhip_ctx *ctx, const mhip_index *ix, uint32_t *perm_out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(ix && perm_out, "null argument");
+    MHIP_NEED(ix->n > 0, "index is empty");
+    hipSetDevice(ctx->device);
+    return check_hip(ctx, hipMemcpyAsync(perm_out, ix->perm, ix->n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                         ctx->stream),
+                     "index order copy");
+}
+
 int mhip_nlist_create(mhip_nlist **out) {
     if (!out) return MHIP_ERR_VALUE;
     *out = new (std::nothrow) mhip_nlist();

@@ -178,17 +178,52 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t *__restrict__ counts, 
     if (tid == 255) totals[blockIdx.x] = s[255];
 }
 
+// Stable scatter of one tile: keys are ranked into tile-local digit order in
+// LDS first, then written out in that order, so that the threads of a wave
+// write runs of consecutive positions of one digit (coalesced) instead of 64
+// scattered 12-byte writes.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ vin, int64_t n,
                                                            int shift, const uint32_t *__restrict__ counts,
                                                            const uint32_t *__restrict__ totals,
                                                            int nblocks, uint64_t *__restrict__ kout,
                                                            uint32_t *__restrict__ vout) {
-    __shared__ uint32_t s_off[256];
+    __shared__ uint64_t s_key[RS_TILE];
+    __shared__ uint32_t s_val[RS_TILE];
+    __shared__ uint32_t s_gb[256];   // global position of the tile's first key of each digit
+    __shared__ uint32_t s_lb[256];   // tile-local position of the first key of each digit
+    __shared__ uint32_t s_run[256];  // running tile-local position per digit
     __shared__ uint32_t s_w[4][256];
     __shared__ uint32_t s_tmp[256];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    // digit base = exclusive scan of totals + this block's scanned count
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int tn = (int)min<int64_t>(RS_TILE, n - base);
+    // the tile's keys in registers, and its digit histogram
+    uint64_t key[RS_ITEMS];
+    uint32_t val[RS_ITEMS];
+    s_run[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int i = r * RS_THREADS + tid;
+        key[r] = i < tn ? kin[base + i] : 0ull;
+        val[r] = i < tn ? vin[base + i] : 0u;
+        if (i < tn) atomicAdd(&s_run[(uint32_t)(key[r] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    // exclusive scans over the digits: tile-local bases and global bases
+    const uint32_t cnt = s_run[tid];
+    s_tmp[tid] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = tid >= off ? s_tmp[tid - off] : 0u;
+        __syncthreads();
+        s_tmp[tid] += v;
+        __syncthreads();
+    }
+    s_lb[tid] = s_tmp[tid] - cnt;
+    s_run[tid] = s_tmp[tid] - cnt;
+    __syncthreads();
     s_tmp[tid] = totals[tid];
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
@@ -197,37 +232,44 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
         s_tmp[tid] += v;
         __syncthreads();
     }
-    s_off[tid] = s_tmp[tid] - totals[tid] + counts[(int64_t)tid * nblocks + blockIdx.x];
+    s_gb[tid] = s_tmp[tid] - totals[tid] + counts[(int64_t)tid * nblocks + blockIdx.x];
     for (int i = tid; i < 1024; i += RS_THREADS) (&s_w[0][0])[i] = 0;
     __syncthreads();
+    // stable ranking into tile-local order: rounds in input order, lanes by
+    // ballot matching within a wave, waves in order within a round
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        int64_t idx = base + r * RS_THREADS + tid;
-        bool valid = idx < n;
-        uint64_t key = valid ? kin[idx] : 0ull;
-        uint32_t val = valid ? vin[idx] : 0u;
-        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const bool valid = r * RS_THREADS + tid < tn;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            bool bit = (d >> b) & 1u;
-            uint64_t m = __ballot(bit);
+        for (int bb = 0; bb < 8; bb++) {
+            const bool bit = (d >> bb) & 1u;
+            const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        uint32_t rank = __popcll(peers & lt_mask);
+        const uint32_t rank = __popcll(peers & lt_mask);
         if (valid && rank == 0) s_w[w][d] = __popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t pos = s_off[d] + rank;
+            uint32_t pos = s_run[d] + rank;
             for (int ww = 0; ww < w; ww++) pos += s_w[ww][d];
-            kout[pos] = key;
-            vout[pos] = val;
+            s_key[pos] = key[r];
+            s_val[pos] = val[r];
         }
         __syncthreads();
-        s_off[tid] += s_w[0][tid] + s_w[1][tid] + s_w[2][tid] + s_w[3][tid];
+        s_run[tid] += s_w[0][tid] + s_w[1][tid] + s_w[2][tid] + s_w[3][tid];
         s_w[0][tid] = s_w[1][tid] = s_w[2][tid] = s_w[3][tid] = 0;
         __syncthreads();
+    }
+    // write out in tile-local digit order: runs of one digit are contiguous
+    for (int i = tid; i < tn; i += RS_THREADS) {
+        const uint64_t k = s_key[i];
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const uint32_t pos = s_gb[d] + (uint32_t)i - s_lb[d];
+        kout[pos] = k;
+        vout[pos] = s_val[i];
     }
 }
 

@@ -96,6 +96,19 @@ class SpatialIndex:
         self.n, self.dim, self.boxsize = n, dim, boxsize
         return self
 
+    def order(self) -> torch.Tensor:
+        """Original row of each sorted position (int64, Morton order)."""
+        perm = torch.empty(self.n, dtype=torch.int32, device=self.pos.device)
+        ctx = context(self.pos.device)
+        raise_for(lib.mhip_index_order(ctx, self.h, ptr(perm)), ctx)
+        return perm.to(torch.int64)
+
+    def bbox(self):
+        """Host copy of the bounding box (lo xyz, hi xyz) as float32."""
+        bb = (ctypes.c_float * 6)()
+        raise_for(lib.mhip_index_info(self.h, None, None, None, bb))
+        return [float(v) for v in bb]
+
     def __del__(self):
         if getattr(self, "h", None):
             lib.mhip_index_destroy(self.h)
