@@ -7,23 +7,28 @@
 //   density = des_ngb*m/(volnorm*h^dim)          src/meshoid/Meshoid.py:169-170
 //
 // Design (one lane per query; 64 consecutive queries in Morton order per
-// wavefront; the tree is a complete 8-ary tree in heap numbering):
-//   main    (1) priming: the leaves holding the wave's own queries; every
-//               lane keeps the k+1 smallest float32 squared distances in
-//               VGPRs, sorted, inserted with v_med3 (one instruction per slot);
-//           (2) wave-uniform depth-first traversal (children near-to-far via
-//               a per-wave LDS stack); node boxes and leaf candidates are read
-//               with scalar loads.  A candidate under its lane's current
-//               (k+1)-th distance goes to that lane's LDS queue; the queues
-//               drain into the sorted lists in batches, so the divergent
-//               insertion runs once per batch, not per candidate.  Leaves where
-//               some lane queued a candidate are recorded;
-//           (3) epilogue: smoothing length on the k distances (safeguarded
-//               Newton, compensated float32 sums), density, and the exact
-//               member set {d2 <= tau} collected from the primed and recorded
-//               leaves only (no second traversal).  Queries whose k-th and
-//               (k+1)-th distances are closer than float32 rounding can order
-//               are flagged.
+// wavefront; the tree is a complete 8-ary tree in heap numbering, root 0,
+// children of x at 8x+1..8x+8, leaves of 16 particles on the last level):
+//   main    Every lane keeps the k+1 smallest float32 squared distances of
+//           its query sorted in VGPRs; an insertion is one in-place v_med3
+//           per slot.  (1) The leaves holding the wave's own queries are
+//           scanned first.  (2) A wave-uniform depth-first walk without a
+//           stack: expanding a node tests its 8 children (one contiguous
+//           block of boxes, scalar loads) against every lane's bound and
+//           keeps the survivors as an 8-bit mask per level in SGPRs; the
+//           child nearest (in index) to the path of the wave's middle query
+//           goes first.  (3) A leaf's 16 candidates (scalar loads) go to each
+//           lane's LDS queue row qn; qn advances when the candidate is under
+//           the lane's (k+1)-th distance.  Queues drain into the sorted lists
+//           in uniform counted loops.  Leaves where some lane queued a
+//           candidate are recorded.  (4) Epilogue: smoothing length
+//           (safeguarded Newton, compensated float32 sums), density, volume,
+//           and the member set {d2 <= tau}: each lane tests the recorded
+//           leaves against its own final radius and scans only those, one
+//           leaf per lane per step (per-lane loads); the indices are staged
+//           in LDS and written as whole rows.  Queries whose k-th and
+//           (k+1)-th distances are closer than float32 rounding can order
+//           are flagged.
 //   pass2   fallback for waves whose leaf record overflowed: per-lane
 //           traversal with the final radius, collecting the members.
 //   resolve for flagged queries only: members in the rounding band around tau
@@ -337,17 +342,6 @@ MHIP_DEV void load_leaf_uniform(const float *sx, const float *sy, const float *s
     }
 }
 
-// node_d2 for a wave-uniform node (scalar loads of the box)
-template <bool PER>
-MHIP_DEV float node_d2_u(const TreeView &t, const QPos &q, uint32_t x) {
-    const int64_t o = (int64_t)x * 8;
-    const float lox = ldc(t.nodes, o), loy = ldc(t.nodes, o + 1), loz = ldc(t.nodes, o + 2);
-    const float hix = ldc(t.nodes, o + 3), hiy = ldc(t.nodes, o + 4), hiz = ldc(t.nodes, o + 5);
-    const float dx = axis_gap<PER>(q.x, lox, hix, t.box, t.slack);
-    const float dy = axis_gap<PER>(q.y, loy, hiy, t.box, t.slack);
-    const float dz = axis_gap<PER>(q.z, loz, hiz, t.box, t.slack);
-    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-}
 
 // squared gap from a lane's query to a box read at wave-uniform addresses
 // (scalar loads); the query interval [ql, qh] per axis is the position widened
