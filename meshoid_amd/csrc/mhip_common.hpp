@@ -205,6 +205,7 @@ struct mhip_nlist {
     int32_t *nlt = nullptr;       // count strictly below tau per slot
     float *h = nullptr;           // smoothing length per slot
     int32_t *status = nullptr;    // per slot
+    uint32_t *amb = nullptr;      // [0] count, [1..] slots flagged ambiguous (for the resolver)
     int masked = 0;               // qsorted/qrow valid (query rows are a subset/reordering)
     int has_d2 = 0;               // d2 filled (needed for export)
     const mhip_index *index = nullptr;

@@ -47,6 +47,7 @@ constexpr float AMBIG_REL = 12.0f * U32;     // fp32 vs fp64 ordering ambiguity 
 constexpr float BAND_REL = 16.0f * U32;      // resolver band half-width
 constexpr float PRUNE_REL = 1.0f + 1e-5f;    // box-test safety factor
 constexpr int BAND_CAP = 32;
+constexpr int64_t RESOLVE_WAVES = 8192;      // resolver grid (one-lane wavefronts first)
 constexpr int32_t ST_REDO = 1 << 16;          // internal: members collected by pass2
 
 struct TreeView {
@@ -209,6 +210,7 @@ struct KnnArgs {
     uint32_t *nl_idx;
     float *nl_d2;
     unsigned long long *stats;  // work counters, may be null
+    uint32_t *amb;              // [0] count, [1..] ambiguous slots
 };
 
 MHIP_DEV int64_t slot_of_lane(int64_t *first) {
@@ -687,6 +689,18 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         st |= ST_REDO;
     }
     if (live) a.st_slot[slot] = st;
+    // ambiguous queries for the resolver, one vector atomic per wavefront by
+    // its lowest active lane (lanes that missed a radius seed left earlier;
+    // in the redo launch only the recomputed lanes are new)
+    const bool amb = live && (st & ST_AMBIGUOUS) && (!a.redo || missed);
+    const uint64_t am = __ballot(amb);
+    if (am) {
+        const int leader = __ffsll((long long)__ballot(1)) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(a.amb, (uint32_t)__popcll(am));
+        base = __shfl(base, leader);
+        if (amb) a.amb[1 + base + __popcll(am & ((1ull << lane) - 1ull))] = (uint32_t)slot;
+    }
 }
 
 // members for lanes whose leaf record overflowed: traversal with radius tau
@@ -735,10 +749,9 @@ MHIP_DEV double exact_d2(const PT *pos, int dim, uint32_t i, uint32_t j, double 
     return s;
 }
 
-// thread per slot; only ambiguous slots do work
+// one ambiguous slot (from the main kernel's list)
 template <bool PER, bool P64, typename PT>
-__global__ __launch_bounds__(64) void k_knn_resolve(KnnArgs a, const PT *__restrict__ pos_in) {
-    const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+MHIP_DEV void resolve_slot(const KnnArgs &a, const PT *__restrict__ pos_in, int64_t slot) {
     if (slot >= a.nq) return;
     if (!(a.st_slot[slot] & ST_AMBIGUOUS)) return;
     const TreeView &t = a.t;
@@ -819,6 +832,24 @@ __global__ __launch_bounds__(64) void k_knn_resolve(KnnArgs a, const PT *__restr
     a.st_slot[slot] = st;
 }
 
+// empties the list of ambiguous slots (a kernel, so graph capture records no
+// memset node)
+__global__ void k_amb_reset(uint32_t *__restrict__ amb) {
+    if (threadIdx.x == 0) amb[0] = 0u;
+}
+
+// the list of ambiguous slots, spread one lane per wavefront first: entry i
+// goes to wavefront i mod W, lane (i / W) mod 64.  Each slot is a long
+// divergent walk, so a short list is best served by many wavefronts with one
+// busy lane each; lanes fill up only when the list is longer than the grid.
+template <bool PER, bool P64, typename PT>
+__global__ __launch_bounds__(64) void k_knn_resolve(KnnArgs a, const PT *__restrict__ pos_in) {
+    const uint32_t count = a.amb[0];
+    const int64_t nw = gridDim.x;
+    for (int64_t i = blockIdx.x + threadIdx.x * nw; i < count; i += 64 * nw)
+        resolve_slot<PER, P64, PT>(a, pos_in, (int64_t)a.amb[1 + i]);
+}
+
 __global__ void k_qsetup_mask(const int64_t *__restrict__ qidx, int64_t nq,
                               const uint32_t *__restrict__ iperm, uint64_t *__restrict__ keys,
                               uint32_t *__restrict__ vals) {
@@ -879,6 +910,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
                         (void **)&nl->nlt,     (void **)&nl->h,    (void **)&nl->status};
         for (auto p : arr)
             if ((rc = nl_grow(ctx, p, nq * 4))) return rc;
+        if ((rc = nl_grow(ctx, (void **)&nl->amb, (nq + 1) * 4))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->idx, (size_t)nq * k * 4))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->d2, (size_t)nq * k * 4))) return rc;
         nl->cap = nq;
@@ -949,9 +981,11 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     a.nl_idx = nl->idx;
     a.nl_d2 = nl->has_d2 ? nl->d2 : nullptr;
     a.stats = ctx->prof >= 2 ? ctx->stats : nullptr;
+    a.amb = nl->amb;
     const bool p64 = ix->pos_f64 != 0;
     const dim3 g((unsigned)((nq + KNN_BLOCK - 1) / KNN_BLOCK));
     hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_amb_reset, dim3(1), dim3(64), 0, s, nl->amb);
     const int need = k + 1;
     ProfRegion pr1(ctx, "knn_main");
     if (need <= 5)
@@ -1007,7 +1041,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     }
     pr2.end();
     ProfRegion pr3(ctx, "knn_resolve");
-    const dim3 gr((unsigned)((nq + 63) / 64));
+    const dim3 gr((unsigned)std::min<int64_t>(RESOLVE_WAVES, (nq + 63) / 64));
     if (per) {
         if (p64)
             hipLaunchKernelGGL((k_knn_resolve<true, true, double>), gr, dim3(64), 0, s, a,
