@@ -304,13 +304,21 @@ MHIP_DEV float hsml_solve(float (&A)[KCAP], int k, int dim, float seed, int32_t 
         else
             lo = h;
         const float dndh = -norm * ds * ih;
-        float hn = h - f / dndh;
-        const bool newton = hn > lo && hn < hi;
-        if (!newton) hn = 0.5f * (lo + hi);
+        const float step = f / dndh;
         // n(h) is C^2 (W is C^2 at q = 1/2 and q = 1), so Newton converges
-        // quadratically: after a step below 1e-4 h the error of hn is at the
-        // float32 noise floor.  A collapsed bracket also ends the solve.
-        if ((newton && fabsf(hn - h) <= 1e-4f * h) || hi - lo <= 2.5e-7f * h) {
+        // quadratically: after a step below 1e-4 h the error of h - step is
+        // at the float32 noise floor.  Such a step ends the solve even when
+        // it rounds onto the bracket end that h itself has just become
+        // (otherwise a converged lane would bisect the whole bracket again
+        // and hold its wavefront for ~15 more iterations).  A collapsed
+        // bracket also ends the solve.
+        if (fabsf(step) <= 1e-4f * h) {
+            h = fminf(fmaxf(h - step, lo), hi);
+            break;
+        }
+        float hn = h - step;
+        if (!(hn > lo && hn < hi)) hn = 0.5f * (lo + hi);
+        if (hi - lo <= 2.5e-7f * h) {
             h = hn;
             break;
         }
