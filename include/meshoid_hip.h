@@ -201,6 +201,39 @@ int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int
 int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
 
 /*
+ * Meshoid.Reconstruct (src/meshoid/Meshoid.py:383-440), in three steps.
+ *
+ * mhip_nearest: the nearest particle of each point, tree.query(points)[1]
+ *   (Meshoid.py:409, k = 1).  points float64 (nt, dim), dim the index's;
+ *   periodic indices wrap the points into [0, boxsize).  The pick is exact in
+ *   float64 (the distance cKDTree computes); at an exact float64 tie the
+ *   lowest particle index wins.  nbr_out int64[nt], original particle index,
+ *   -1 for a point with non-finite coordinates.
+ *
+ * mhip_unique_inverse: np.unique(idx, return_inverse=True) (Meshoid.py:410)
+ *   for idx int64[nt] in [0, n): uniq_out int64[n] (capacity) receives the
+ *   sorted distinct indices, inv_out int64[nt] the position of each idx in
+ *   uniq_out, *nuniq (host) their count.  Synchronises the stream.  An idx
+ *   outside [0, n) is an error.
+ *
+ * mhip_reconstruct: f_target (Meshoid.py:413-440) for order 0 (f[nbr]), 1
+ *   (+ dx . D) or 2 (3D only: + 0.5 dx_a^2 D2_aa + dx_a dx_(a+1) D2_a,a+1, the
+ *   reference's term order).  pos: the particle positions (float32/float64,
+ *   (n, dim)); dx = point - pos[nbr], nearest image when boxsize > 0.  f (n, K)
+ *   float32/float64; d1 float32 (nuniq, K, dim) and d2 float32 (nuniq, K, 6)
+ *   are D(f, order) and D2(f) on the rows uniq (NULL when the order does not
+ *   use them); out (nt, K) in f's dtype, every partial sum rounded to it as
+ *   the reference's in-place additions do.
+ */
+int mhip_nearest(mhip_ctx *ctx, const mhip_index *ix, const double *points, int64_t nt, int dim,
+                 int64_t *nbr_out);
+int mhip_unique_inverse(mhip_ctx *ctx, const int64_t *idx, int64_t nt, int64_t n, int64_t *uniq_out,
+                        int64_t *inv_out, int64_t *nuniq);
+int mhip_reconstruct(mhip_ctx *ctx, const double *points, int64_t nt, int dim, double boxsize, const void *pos,
+                     int pos_f64, const int64_t *nbr, const int64_t *inv, const void *f, int f_f64, int K,
+                     const float *d1, const float *d2, int order, void *out);
+
+/*
  * Batched form (no counterpart in the reference, which runs one Meshoid at a
  * time: Meshoid.py:15-170 then D at :237-268 per snapshot).  nsnap particle
  * sets of n[s] float32 positions (n[s], 3) each, with des_ngb k[s] and K

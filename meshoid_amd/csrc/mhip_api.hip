@@ -338,4 +338,30 @@ int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, in
     return integrate(ctx, vol, f, f_f64 ? 1 : 0, n, K, out);
 }
 
+int mhip_nearest(mhip_ctx *ctx, const mhip_index *ix, const double *points, int64_t nt, int dim,
+                 int64_t *nbr_out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(ix && nbr_out && (points || nt == 0), "null argument");
+    hipSetDevice(ctx->device);
+    return nearest_run(ctx, ix, points, nt, dim, nbr_out);
+}
+
+int mhip_unique_inverse(mhip_ctx *ctx, const int64_t *idx, int64_t nt, int64_t n, int64_t *uniq_out,
+                        int64_t *inv_out, int64_t *nuniq) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(nuniq && ((idx && uniq_out && inv_out) || nt == 0), "null argument");
+    hipSetDevice(ctx->device);
+    return unique_inverse(ctx, idx, nt, n, uniq_out, inv_out, nuniq);
+}
+
+int mhip_reconstruct(mhip_ctx *ctx, const double *points, int64_t nt, int dim, double boxsize, const void *pos,
+                     int pos_f64, const int64_t *nbr, const int64_t *inv, const void *f, int f_f64, int K,
+                     const float *d1, const float *d2, int order, void *out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED((points && pos && nbr && f && out && (inv || order == 0)) || nt == 0, "null argument");
+    hipSetDevice(ctx->device);
+    return reconstruct(ctx, points, nt, dim, boxsize, pos, pos_f64 ? 1 : 0, nbr, inv, f, f_f64 ? 1 : 0, K, d1, d2,
+                       order, out);
+}
+
 }  // extern "C"

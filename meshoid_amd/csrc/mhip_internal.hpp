@@ -40,6 +40,16 @@ size_t mlp_blob_bytes();
 
 int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out);
 
+// nearest particle of arbitrary points (mhip_knn.hip) and the reconstruction
+// around it (mhip_recon.hip)
+int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int64_t nt, int dim,
+                int64_t *out);
+int unique_inverse(mhip_ctx *ctx, const int64_t *nbr, int64_t nt, int64_t n, int64_t *uniq, int64_t *inv,
+                   int64_t *nuniq);
+int reconstruct(mhip_ctx *ctx, const double *targets, int64_t nt, int dim, double box, const void *pos,
+                int pos_f64, const int64_t *nbr, const int64_t *inv, const void *f, int f_f64, int K,
+                const float *d1, const float *d2, int order, void *out);
+
 int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows,
               int K, int order, int weighted, float *out1, float *out2, int32_t *status);
 

@@ -904,6 +904,27 @@ static void launch_main(const KnnArgs &a, bool per, bool p64, dim3 g, hipStream_
     }
 }
 
+static TreeView tree_view(const mhip_index *ix) {
+    TreeView t;
+    t.sx = ix->sx;
+    t.sy = ix->sy;
+    t.sz = ix->sz;
+    t.lx = ix->lx;
+    t.ly = ix->ly;
+    t.lz = ix->lz;
+    t.nodes = ix->nodes;
+    t.first_leaf = ix->first_leaf;
+    t.nleaf = ix->nleaf;
+    t.depth = ix->depth;
+    const bool per = ix->box > 0;
+    t.box = per ? (float)ix->box : 0.f;
+    t.half = per ? 0.5f * (float)ix->box : 0.f;
+    float m = per ? (float)ix->box : 0.f;
+    for (int c = 0; c < 6; c++) m = fmaxf(m, fabsf(ix->bbox[c]));
+    t.slack = ix->pos_f64 ? 2.4e-7f * m : 0.f;  // 4 ulp of the largest |coordinate|
+    return t;
+}
+
 int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, int k,
             const double *m_rows, const float *seed_h, const float *seed_r, float *h_out,
             float *rho_out, float *vol_out, int32_t *status_out, int flags, mhip_nlist *nl) {
@@ -949,24 +970,8 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
                            ctx->stream, ks, vs, nq, nl->qsorted, nl->qrow);
     }
     KnnArgs a;
-    a.t.sx = ix->sx;
-    a.t.sy = ix->sy;
-    a.t.sz = ix->sz;
-    a.t.lx = ix->lx;
-    a.t.ly = ix->ly;
-    a.t.lz = ix->lz;
-    a.t.nodes = ix->nodes;
-    a.t.first_leaf = ix->first_leaf;
-    a.t.nleaf = ix->nleaf;
-    a.t.depth = ix->depth;
+    a.t = tree_view(ix);
     const bool per = ix->box > 0;
-    a.t.box = per ? (float)ix->box : 0.f;
-    a.t.half = per ? 0.5f * (float)ix->box : 0.f;
-    {
-        float m = per ? (float)ix->box : 0.f;
-        for (int c = 0; c < 6; c++) m = fmaxf(m, fabsf(ix->bbox[c]));
-        a.t.slack = ix->pos_f64 ? 2.4e-7f * m : 0.f;  // 4 ulp of the largest |coordinate|
-    }
     a.qsorted = masked ? nl->qsorted : nullptr;
     a.qrow = masked ? nl->qrow : nullptr;
     a.perm = ix->perm;
@@ -1106,6 +1111,193 @@ int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *d
                        nl->idx, nl->d2, nl->masked ? nl->qrow : nullptr, ix->perm, nl->nq, nl->k,
                        ngb_out, dist_out);
     return check_hip(ctx, hipGetLastError(), "export launch");
+}
+
+// ---------------------------------------------------------------------------
+// nearest particle of arbitrary float64 points: cKDTree.query(x)[1], k = 1
+// (src/meshoid/Meshoid.py:409).  One thread per point: (1) a greedy descent
+// to the leaf whose box is nearest gives a first bound; (2) a depth-first
+// walk over every node within that bound (widened by the float32 rounding
+// band) keeps the nearest and the second-nearest float32 d2; (3) when those
+// two lie closer than float32 rounding can order, the band is re-ranked with
+// float64 distances from the caller's positions (the distance cKDTree
+// computes), ties to the lowest original index.  The point is split into
+// float32 hi + lo parts, so its float32 distances are as accurate as those
+// between two particles.
+// ---------------------------------------------------------------------------
+struct NearestArgs {
+    TreeView t;
+    const uint32_t *perm;
+    const void *pos_in;
+    int64_t n;
+    int dim;
+    double box;        // > 0: periodic in [0, box)
+    const double *tg;  // (nt, dim) row-major
+    int64_t nt;
+    int64_t *out;      // original particle index per point, -1 for a non-finite point
+};
+
+template <typename PT, bool PER>
+MHIP_DEV double exact_d2_point(const PT *pos, int dim, uint32_t j, const double (&tp)[3], double box) {
+    double s = 0.0;
+    for (int c = 0; c < dim; c++) {
+        double d = (double)pos[(int64_t)j * dim + c] - tp[c];
+        if (PER) d = wrap<double>(d, box);
+        s = __dadd_rn(s, __dmul_rn(d, d));
+    }
+    return s;
+}
+
+template <bool PER, bool P64>
+MHIP_DEV void nearest_scan(const TreeView &t, const QPos &q, int64_t leaf, float &best, float &second,
+                           uint32_t &bp) {
+    float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
+    load_leaf<P64>(t, leaf, cx, cy, cz, clx, cly, clz);
+#pragma unroll
+    for (int j = 0; j < LEAF; j++) {
+        // padding particles carry NaN coordinates: never below either bound
+        const float d2 = dist2<PER, true>(t, q, cx[j], cy[j], cz[j], clx[j], cly[j], clz[j]);
+        if (d2 < best) {
+            second = best;
+            best = d2;
+            bp = (uint32_t)(leaf * LEAF + j);
+        } else if (d2 < second) {
+            second = d2;
+        }
+    }
+}
+
+template <bool PER, bool P64, typename PT>
+__global__ __launch_bounds__(256) void k_nearest(NearestArgs a) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.nt) return;
+    double tp[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < a.dim; c++) {
+        double v = a.tg[i * a.dim + c];
+        if (PER && (v < 0.0 || v >= a.box)) {
+            v -= floor(v / a.box) * a.box;
+            if (!(v >= 0.0 && v < a.box)) v = 0.0;  // rounded onto the upper edge
+        }
+        tp[c] = v;
+    }
+    QPos q;
+    q.x = (float)tp[0];
+    q.y = (float)tp[1];
+    q.z = (float)tp[2];
+    q.lx = (float)(tp[0] - (double)q.x);
+    q.ly = (float)(tp[1] - (double)q.y);
+    q.lz = (float)(tp[2] - (double)q.z);
+    // box tests use the hi parts: the slack covers the point's lo parts too
+    TreeView t = a.t;
+    t.slack = a.t.slack + fmaxf(fabsf(q.lx), fmaxf(fabsf(q.ly), fabsf(q.lz)));
+    const uint32_t first_leaf = (uint32_t)t.first_leaf;
+    // (1) greedy descent (a non-empty node always has a nearest non-empty child)
+    uint32_t x = 0;
+    while (x < first_leaf) {
+        const uint32_t c0 = FANOUT * x + 1;
+        uint32_t bc = c0;
+        float bd = node_d2<PER>(t, q, c0);
+        for (uint32_t c = 1; c < FANOUT; c++) {
+            const float d = node_d2<PER>(t, q, c0 + c);
+            if (d < bd) {
+                bd = d;
+                bc = c0 + c;
+            }
+        }
+        x = bc;
+    }
+    const uint32_t gl = x;
+    float best = INFINITY, second = INFINITY;
+    uint32_t bp = 0xffffffffu;
+    if (gl - first_leaf < t.nleaf) nearest_scan<PER, P64>(t, q, gl - first_leaf, best, second, bp);
+    // (2) every other leaf within the bound
+    x = 0;
+    bool more = true;
+    while (more) {
+        if (node_d2<PER>(t, q, x) <= best * ((1.0f + BAND_REL) * PRUNE_REL)) {
+            if (x < first_leaf) {
+                x = FANOUT * x + 1;
+                continue;
+            }
+            if (x != gl && x - first_leaf < t.nleaf) nearest_scan<PER, P64>(t, q, x - first_leaf, best, second, bp);
+        }
+        more = next_after(x);
+    }
+    int64_t res = -1;
+    if (bp != 0xffffffffu) {
+        res = a.perm[bp];
+        if (second <= best * (1.0f + AMBIG_REL)) {
+            // (3) float64 re-rank of the rounding band
+            const PT *pos = (const PT *)a.pos_in;
+            const float lim = best * (1.0f + BAND_REL);
+            double be = INFINITY;
+            uint32_t bo = 0xffffffffu;
+            x = 0;
+            more = true;
+            while (more) {
+                if (node_d2<PER>(t, q, x) <= lim * PRUNE_REL) {
+                    if (x < first_leaf) {
+                        x = FANOUT * x + 1;
+                        continue;
+                    }
+                    if (x - first_leaf < t.nleaf) {
+                        const int64_t leaf = x - first_leaf;
+                        float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
+                        load_leaf<P64>(t, leaf, cx, cy, cz, clx, cly, clz);
+                        for (int j = 0; j < LEAF; j++) {
+                            const int64_t p = leaf * LEAF + j;
+                            const float d2 = dist2<PER, true>(t, q, cx[j], cy[j], cz[j], clx[j], cly[j], clz[j]);
+                            if (d2 <= lim && p < a.n) {
+                                const uint32_t o = a.perm[p];
+                                const double e = exact_d2_point<PT, PER>(pos, a.dim, o, tp, a.box);
+                                if (e < be || (e == be && o < bo)) {
+                                    be = e;
+                                    bo = o;
+                                }
+                            }
+                        }
+                    }
+                }
+                more = next_after(x);
+            }
+            res = bo;
+        }
+    }
+    a.out[i] = res;
+}
+
+int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int64_t nt, int dim,
+                int64_t *out) {
+    if (!ix || ix->n < 1) return fail(ctx, -1, "the index holds no particles");
+    if (dim != ix->dim) return fail(ctx, -1, "point dimension differs from the particles'");
+    if (nt < 0) return fail(ctx, -1, "negative point count");
+    if (nt == 0) return 0;
+    NearestArgs a;
+    a.t = tree_view(ix);
+    a.perm = ix->perm;
+    a.pos_in = ix->pos_in;
+    a.n = ix->n;
+    a.dim = ix->dim;
+    a.box = ix->box > 0 ? ix->box : 0.0;
+    a.tg = targets;
+    a.nt = nt;
+    a.out = out;
+    const bool per = ix->box > 0, p64 = ix->pos_f64 != 0;
+    const dim3 g((unsigned)((nt + 255) / 256));
+    hipStream_t s = ctx->stream;
+    ProfRegion pr(ctx, "nearest");
+    if (per) {
+        if (p64)
+            hipLaunchKernelGGL((k_nearest<true, true, double>), g, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_nearest<true, false, float>), g, dim3(256), 0, s, a);
+    } else {
+        if (p64)
+            hipLaunchKernelGGL((k_nearest<false, true, double>), g, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_nearest<false, false, float>), g, dim3(256), 0, s, a);
+    }
+    return check_hip(ctx, hipGetLastError(), "nearest launch");
 }
 
 }  // namespace mhip

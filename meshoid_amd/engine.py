@@ -300,3 +300,62 @@ def integrate(vol: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
     raise_for(lib.mhip_integrate(ctx, ptr(vol), ptr(f2), int(f.dtype == torch.float64), f2.shape[0], K,
                                  ptr(out)), ctx)
     return out
+
+
+def nearest(index: SpatialIndex, points: torch.Tensor) -> torch.Tensor:
+    """Nearest particle of each point, cKDTree.query(points)[1]
+    (src/meshoid/Meshoid.py:409): points (nt, dim) float64 device tensor;
+    int64 (nt,) original particle indices, exact in float64."""
+    _need(points, (torch.float64,), "target_points")
+    if points.dim() == 1:
+        points = points[:, None]
+    nt, dim = points.shape
+    if dim != index.dim:
+        raise ValueError(f"target_points have dimension {dim}, the particles {index.dim}")
+    out = torch.empty(nt, dtype=torch.int64, device=points.device)
+    ctx = context(points.device)
+    raise_for(lib.mhip_nearest(ctx, index.h, ptr(points), nt, dim, ptr(out)), ctx)
+    return out
+
+
+def unique_inverse(idx: torch.Tensor, n: int):
+    """np.unique(idx, return_inverse=True) for int64 indices in [0, n)
+    (src/meshoid/Meshoid.py:410): (sorted distinct indices, inverse)."""
+    _need(idx, (torch.int64,), "indices")
+    nt = idx.numel()
+    uniq = torch.empty(max(1, min(n, nt)), dtype=torch.int64, device=idx.device)
+    inv = torch.empty(nt, dtype=torch.int64, device=idx.device)
+    cnt = ctypes.c_int64()
+    ctx = context(idx.device)
+    raise_for(lib.mhip_unique_inverse(ctx, ptr(idx), nt, int(n), ptr(uniq) if nt else None,
+                                      ptr(inv) if nt else None, ctypes.byref(cnt)), ctx)
+    return uniq[: cnt.value], inv
+
+
+def reconstruct(points, pos, boxsize, nbr, inv, f, d1=None, d2=None, order=1):
+    """f_target of Meshoid.Reconstruct (src/meshoid/Meshoid.py:413-440):
+    points (nt, dim) float64, pos (n, dim), nbr/inv int64 (nt,), f (n, K)
+    float32/float64, d1 (nu, K, dim) / d2 (nu, K, 6) float32 on the unique
+    rows; returns (nt, K) in f's dtype."""
+    _need(points, (torch.float64,), "target_points")
+    _need(pos, (torch.float32, torch.float64), "pos")
+    _need(f, (torch.float32, torch.float64), "f")
+    _need(nbr, (torch.int64,), "nbr")
+    _need(inv, (torch.int64,), "inv")
+    _need(d1, (torch.float32,), "d1")
+    _need(d2, (torch.float32,), "d2")
+    if points.dim() == 1:
+        points = points[:, None]
+    nt, dim = points.shape
+    K = f.shape[1] if f.dim() > 1 else 1
+    if order >= 1 and d1 is None:
+        raise ValueError("order >= 1 needs first derivatives")
+    if order == 2 and d2 is None:
+        raise ValueError("order 2 needs second derivatives")
+    out = torch.empty((nt, K), dtype=f.dtype, device=f.device)
+    ctx = context(f.device)
+    raise_for(lib.mhip_reconstruct(ctx, ptr(points), nt, dim, float(boxsize) if boxsize else -1.0, ptr(pos),
+                                   int(pos.dtype == torch.float64), ptr(nbr), ptr(inv), ptr(f),
+                                   int(f.dtype == torch.float64), K, ptr(d1), ptr(d2), int(order), ptr(out)),
+              ctx)
+    return out

@@ -336,10 +336,8 @@ class Meshoid:
                 raise IndexError(f"index {int(np.max(self.particle_mask))} is out of bounds for axis 0 with size {nf}")
         return fd, shape[1:], is_t
 
-    def _apply(self, f, order, weighted, second):
-        if self._nl is None:
-            self.TreeUpdate()
-        fd, tail, is_t = self._field(f)
+    def _apply_dev(self, fd, order, weighted, second):
+        """(Nmask, K, nd) float32 device derivatives of the (nf, K) field."""
         w = self._weighted.setdefault(order, bool(weighted))
         if self.verbose and order not in self._dw:
             print(f"Computing weights for derivatives of order {order}...")
@@ -347,7 +345,13 @@ class Meshoid:
                                     want_d2=second)
         if bool(((st & ST_SINGULAR) != 0).any()):
             raise np.linalg.LinAlgError("Matrix is singular to machine precision.")
-        out = d2 if second else d1
+        return d2 if second else d1
+
+    def _apply(self, f, order, weighted, second):
+        if self._nl is None:
+            self.TreeUpdate()
+        fd, tail, is_t = self._field(f)
+        out = self._apply_dev(fd, order, weighted, second)
         out = out.reshape((self.Nmask,) + tuple(tail) + (out.shape[-1],))
         if is_t:
             return out
@@ -378,3 +382,109 @@ class Meshoid:
         """Divergence of a vector field (Meshoid.py:317-331)."""
         dv = self.D(v[self.particle_mask])
         return dv[:, 0, 0] + dv[:, 1, 1] + dv[:, 2, 2]
+
+    # ---------------------------------------------------- reconstruction
+    def _set_mask(self, idx):
+        self.particle_mask = idx
+        self.Nmask = len(idx)
+        self._identity_mask = self.Nmask == self.N and np.array_equal(idx, np.arange(self.N))
+
+    def Reconstruct(self, f, target_points, order: int = 1):
+        """Values of f at arbitrary points from each point's nearest particle
+        (Meshoid.py:383-440): order 0 its value, 1 plus dx . D(f), 2 (3D) plus
+        the second-order terms of D2(f).  As in the reference, order >= 1 sets
+        ``particle_mask`` to the sorted distinct nearest particles and re-runs
+        ``TreeUpdate`` on them when that changes it.  Returns (N_target, ...)
+        in f's dtype."""
+        if self.tree is None:
+            self.BuildTree()
+        dev = self._pos_d.device
+        tp = target_points
+        tp_d = tp.to(dev, torch.float64) if isinstance(tp, torch.Tensor) else \
+            torch.from_numpy(np.ascontiguousarray(tp, dtype=np.float64)).to(dev)
+        single = tp_d.dim() == 1 and self.dim > 1
+        tp_d = tp_d.reshape(-1, self.dim if single or tp_d.dim() == 1 else tp_d.shape[-1]).contiguous()
+        nbr = engine.nearest(self.tree, tp_d)
+        is_t = isinstance(f, torch.Tensor)
+        tail = tuple(f.shape[1:])
+        # f keeps its dtype (order 0 returns np.take(f, ...) of any dtype)
+        fd = (f.to(dev) if is_t else torch.from_numpy(np.ascontiguousarray(f)).to(dev)).contiguous()
+        nf = fd.shape[0]
+        fd = fd.reshape(nf, -1).contiguous()
+        floating = fd.dtype in (torch.float32, torch.float64)
+        if nbr.numel() and nf < self.N and int(nbr.max()) >= nf:
+            raise IndexError(f"index {int(nbr.max())} is out of bounds for axis 0 with size {nf}")
+
+        def done(out):
+            out = out.reshape((tp_d.shape[0],) + tail)
+            if single:
+                out = out[0]
+            return out if is_t else out.cpu().numpy()
+
+        if order == 0:
+            if not floating:  # np.take of any dtype
+                if nbr.numel() and int(nbr.min()) < 0:
+                    raise ValueError("target_points must be finite")
+                return done(fd[nbr])
+            inv = torch.zeros_like(nbr)
+            return done(engine.reconstruct(tp_d, self._pos_d, self.boxsize, nbr, inv, fd, order=0))
+        if order not in (1, 2):
+            raise ValueError("order must be 0, 1 or 2")
+        uniq, inv = engine.unique_inverse(nbr, self.N)
+        u = uniq.cpu().numpy()
+        if not np.array_equal(self.particle_mask, u):
+            self._set_mask(u)
+            self.TreeUpdate()  # neighbour lists of just the target neighbours
+            self.reset_dweights()
+        if not floating:
+            # the reference's in-place f_target += float64 fails the same way
+            raise TypeError(f"Cannot cast ufunc 'add' output from dtype('float64') to "
+                            f"dtype('{np.dtype(str(fd.dtype).split('.')[-1])}') with casting rule 'same_kind'")
+        if order == 2 and self.dim != 3:
+            raise ValueError("order-2 reconstruction is defined for 3D particles only")
+        if self._nl is None:
+            self.TreeUpdate()
+        d1 = self._apply_dev(fd, order, True, second=False).contiguous()
+        d2 = self._apply_dev(fd, 2, True, second=True).contiguous() if order == 2 else None
+        return done(engine.reconstruct(tp_d, self._pos_d, self.boxsize, nbr, inv, fd, d1, d2, order))
+
+    def Slice(self, f, size=None, plane="z", center=None, res: int = 128, order: int = 1,
+              return_grid: bool = False):
+        """f reconstructed on a res x res grid in a plane through ``center``
+        (Meshoid.py:442-502): the grid spans [-size/2, size/2] (end points
+        included) along the two in-plane axes, normal ``plane`` ('x', 'y',
+        otherwise z)."""
+        if center is None:
+            center = self.center
+        if size is None:
+            size = self.L
+        if self.tree is None:
+            self.TreeUpdate()
+        ax = np.linspace(-size / 2, size / 2, res)
+        x, y = np.meshgrid(ax, ax, indexing="ij")
+        u, v, w = x.ravel(), y.ravel(), np.zeros(res * res)
+        cols = {"x": (w, u, v), "y": (u, w, v)}.get(plane, (u, v, w))
+        grid = np.column_stack(cols) + center
+        f_grid = self.Reconstruct(f, grid, order)
+        shape = (res, res) + tuple(f.shape[1:])
+        f_grid = f_grid.reshape(shape)
+        if return_grid:
+            return x, y, f_grid
+        return f_grid
+
+    def InterpToGrid(self, f, size=None, center=None, res: int = 128, order: int = 1,
+                     return_grid: bool = False):
+        """f reconstructed at the cell centres of a res^3 grid of side
+        ``size`` around ``center`` (Meshoid.py:504-555)."""
+        if center is None:
+            center = self.center
+        if size is None:
+            size = self.L
+        edges = np.linspace(-size / 2, size / 2, res + 1)
+        mid = (edges[1:] + edges[:-1]) / 2
+        X, Y, Z = np.meshgrid(mid, mid, mid, indexing="ij")
+        grid = np.column_stack((X.ravel(), Y.ravel(), Z.ravel())) + center
+        f_grid = self.Reconstruct(f, grid, order).reshape((res, res, res) + tuple(f.shape[1:]))
+        if return_grid:
+            return X, Y, Z, f_grid
+        return f_grid

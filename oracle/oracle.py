@@ -19,6 +19,9 @@ meshoid source tree):
 * :func:`kernel_weights`, :func:`kernel_average`, :func:`kernel_variance`,
   :func:`integrate` -- ``get_kernel_weights``, ``KernelAverage``,
   ``KernelVariance`` and ``Integrate``, ``src/meshoid/Meshoid.py:172-177,333-381``.
+* :meth:`OracleMeshoid.Reconstruct`, ``Slice``, ``InterpToGrid`` --
+  ``src/meshoid/Meshoid.py:383-555`` (the nearest particle from :func:`knn`
+  with k = 1, ties to the lowest index).
 
 The numerical core is ``meshoid_oracle.c`` (strict FP64, no FMA contraction),
 built into ``liboracle.so`` next to this file.
@@ -253,3 +256,61 @@ class OracleMeshoid:
     def Div(self, v):
         dv = self.D(v[self.particle_mask])
         return dv[:, 0, 0] + dv[:, 1, 1] + dv[:, 2, 2]
+
+    # ----------------------------------------------------- reconstruction
+    def Reconstruct(self, f, target_points, order=1):
+        """Meshoid.py:383-440.  f_target starts as f at the nearest particle
+        (in f's dtype) and each term is added in place, so float32 f rounds
+        after the first-order term, the pure second-order terms and each mixed
+        term."""
+        tp = _pos2d(target_points)
+        if self.boxsize:
+            tp_q = np.mod(tp, self.boxsize)
+        else:
+            tp_q = tp
+        nearest = knn(self.pos, 1, self.boxsize, queries=tp_q)[1][:, 0]
+        uniq, inv = np.unique(nearest, return_inverse=True)
+        f = np.asarray(f)
+        out = f[nearest].copy()
+        if order == 0:
+            return out
+        dx = tp - np.asarray(self.pos, dtype=np.float64)[nearest]
+        if self.boxsize:
+            L = float(self.boxsize)
+            far = np.abs(dx) > L / 2
+            dx = np.where(far, -np.sign(dx) * (L - np.abs(dx)), dx)
+        if not np.array_equal(self.particle_mask, uniq):
+            self.particle_mask = uniq
+            self.Nmask = len(uniq)
+            self.TreeUpdate()
+            self.dweights = self.d2weights = self.dweights_3rdorder = None
+        g = self.D(f, order=order)[inv]  # (nt, ..., dim)
+        lin = sum(dx[:, d].reshape((-1,) + (1,) * (g.ndim - 2)) * g[..., d] for d in range(self.dim))
+        out += lin  # float32 f: one rounding of (f + lin) computed in float64
+        if order == 1:
+            return out
+        h = self.D2(f)[inv]  # (nt, ..., 6): xx, yy, zz, xy, yz, zx
+        col = lambda a: a.reshape((-1,) + (1,) * (h.ndim - 2))
+        pure = sum(col(dx[:, d] * dx[:, d]) * h[..., d] for d in range(3))
+        out += 0.5 * pure
+        for d in range(3):
+            out += col(dx[:, d] * dx[:, (d + 1) % 3]) * h[..., 3 + d]
+        return out
+
+    def Slice(self, f, size, center, res=128, order=1, plane="z"):
+        """Meshoid.py:442-502 (size and center given)."""
+        ax = np.linspace(-size / 2, size / 2, res)
+        x, y = np.meshgrid(ax, ax, indexing="ij")
+        z = np.zeros(res * res)
+        cols = {"x": (z, x.ravel(), y.ravel()), "y": (x.ravel(), z, y.ravel())}.get(
+            plane, (x.ravel(), y.ravel(), z))
+        grid = np.stack(cols, axis=1) + center
+        return self.Reconstruct(f, grid, order).reshape((res, res) + np.shape(f)[1:])
+
+    def InterpToGrid(self, f, size, center, res=128, order=1):
+        """Meshoid.py:504-555 (size and center given): cell centres."""
+        e = np.linspace(-size / 2, size / 2, res + 1)
+        c = 0.5 * (e[1:] + e[:-1])
+        X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+        grid = np.stack((X.ravel(), Y.ravel(), Z.ravel()), axis=1) + center
+        return self.Reconstruct(f, grid, order).reshape((res, res, res) + np.shape(f)[1:])

@@ -1,0 +1,192 @@
+"""Reconstruct / Slice / InterpToGrid (src/meshoid/Meshoid.py:383-555) on the
+GPU against the FP64 oracle (tests/test_recon_oracle.py pins the oracle):
+
+* the nearest particle of each point equals the oracle's exact float64 kNN
+  with k = 1 (ties to the lowest index, as the oracle orders them), for
+  float32 and float64 particles, periodic boxes, points outside the particles'
+  box and points exactly on particles;
+* np.unique(..., return_inverse=True) of the device kernels bit for bit;
+* order 0 bit for bit; orders 1 and 2 to the derivative tolerance; the
+  particle_mask side effect equal to the oracle's;
+* Slice and InterpToGrid grids equal to the oracle's.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _nearest_case(torch, pos, tp, boxsize=None):
+    from meshoid_amd import engine
+
+    ix = engine.SpatialIndex().build(torch.from_numpy(pos).cuda(), boxsize)
+    return engine.nearest(ix, torch.from_numpy(np.ascontiguousarray(tp, np.float64)).cuda()).cpu().numpy()
+
+
+def test_nearest_matches_oracle(oracle):
+    torch = _torch()
+    from meshoid_amd import synthetic
+
+    rng = np.random.default_rng(0)
+    cases = []
+    p = rng.random((20000, 3))
+    cases.append(("uniform f64", p, rng.random((5000, 3)) * 1.4 - 0.2, None))
+    pc = synthetic.clustered(50000, seed=1, nclump=16, box=10.0, background=0.1)
+    tc = np.concatenate([rng.random((3000, 3)) * 10.0, pc[:2000].astype(np.float64) + 1e-7,
+                         pc[2000:3000].astype(np.float64)])  # on particles: distance 0
+    cases.append(("clustered f32", pc, tc, None))
+    pp = rng.random((20000, 3)).astype(np.float32) * 5.0
+    tpp = rng.random((4000, 3)) * 5.0
+    tpp[:500, 0] = rng.random(500) * 1e-3  # next to the x = 0 face
+    cases.append(("periodic f32", pp, tpp, 5.0))
+    p2 = rng.random((8000, 2))
+    cases.append(("2d f64", p2, rng.random((2000, 2)), None))
+    for name, pos, tp, box in cases:
+        say("nearest:", name, pos.shape, tp.shape)
+        got = _nearest_case(torch, pos, tp, box)
+        q = np.mod(tp, box) if box else tp
+        want = oracle.knn(pos.astype(np.float64), 1, box, queries=q)[1][:, 0]
+        bad = np.nonzero(got != want)[0]
+        assert bad.size == 0, (name, bad[:5], got[bad[:5]], want[bad[:5]])
+
+
+def test_nearest_ties_and_non_finite():
+    torch = _torch()
+    from meshoid_amd import engine
+
+    # a lattice: points at cell centres are equidistant from 8 particles
+    g = np.arange(8.0)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    tp = pos[:50] + 0.5
+    got = _nearest_case(torch, pos, tp)
+    d2 = ((tp[:, None, :] - pos[None]) ** 2).sum(-1)
+    assert np.array_equal(got, d2.argmin(1))  # argmin: first (lowest) index on ties
+    ix = engine.SpatialIndex().build(torch.from_numpy(pos).cuda())
+    bad = torch.tensor([[np.nan, 0.0, 0.0], [1.0, 1.0, 1.0]], dtype=torch.float64, device="cuda")
+    nb = engine.nearest(ix, bad)
+    assert nb.cpu().tolist()[0] == -1
+    with pytest.raises(ValueError):
+        engine.unique_inverse(nb, len(pos))
+
+
+def test_unique_inverse_matches_numpy():
+    torch = _torch()
+    from meshoid_amd import engine
+
+    rng = np.random.default_rng(3)
+    for n, nt in [(1, 5), (100, 1), (5000, 20000), (300000, 70000), (2100, 2100)]:
+        idx = rng.integers(0, n, nt)
+        u, inv = engine.unique_inverse(torch.from_numpy(idx).cuda(), n)
+        uw, iw = np.unique(idx, return_inverse=True)
+        assert np.array_equal(u.cpu().numpy(), uw) and np.array_equal(inv.cpu().numpy(), iw), (n, nt)
+    u, inv = engine.unique_inverse(torch.empty(0, dtype=torch.int64, device="cuda"), 10)
+    assert u.numel() == 0 and inv.numel() == 0
+
+
+def _pair(oracle, pos, **kw):
+    import meshoid_amd
+
+    return meshoid_amd.Meshoid(pos, **kw), oracle.OracleMeshoid(pos, **kw)
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+
+
+def test_reconstruct_orders_match_oracle(oracle):
+    _torch()
+    rng = np.random.default_rng(4)
+    pos = rng.random((30000, 3))
+    x = pos
+    f = np.stack([np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1]) + x[:, 2] ** 2,
+                  x[:, 0] * x[:, 1] - 0.5 * x[:, 2]], axis=1)
+    tp = rng.random((4000, 3)) * 0.9 + 0.05
+    for order in (0, 1, 2):
+        say("reconstruct order", order)
+        M, O = _pair(oracle, pos, des_ngb=32)
+        r = M.Reconstruct(f, tp, order)
+        ro = O.Reconstruct(f, tp, order)
+        assert r.shape == ro.shape == (4000, 2) and r.dtype == np.float64
+        if order == 0:
+            assert np.array_equal(r, ro)
+            assert np.array_equal(M.particle_mask, np.arange(30000))
+            continue
+        assert np.array_equal(M.particle_mask, O.particle_mask)
+        assert np.array_equal(M.SmoothingLength().shape, O.kernel_radius.shape)
+        e = _rel(r, ro)
+        say("  max rel", e.max(), "median", np.median(e))
+        if order == 1:
+            assert e.max() <= 1e-5
+        else:
+            # order-2 rows with a rank-deficient 9x9 moment matrix are noise in
+            # both (the GPU flags them NaN): compare the rest
+            ok = np.isfinite(r).all(1)
+            assert ok.mean() > 0.99
+            assert np.quantile(e[ok], 0.99) <= 1e-5
+
+
+def test_reconstruct_float32_field_and_linear_exactness(oracle):
+    _torch()
+    rng = np.random.default_rng(5)
+    pos = rng.random((20000, 3)).astype(np.float32)
+    A = rng.standard_normal((3, 3))
+    f = (pos.astype(np.float64) @ A).astype(np.float32)
+    tp = rng.random((3000, 3)) * 0.8 + 0.1
+    M, O = _pair(oracle, pos, des_ngb=32)
+    r = M.Reconstruct(f, tp, 1)
+    ro = O.Reconstruct(f, tp, 1)
+    assert r.dtype == np.float32 and ro.dtype == np.float32
+    assert np.abs(r.astype(np.float64) - tp @ A).max() < 2e-5
+    assert np.abs(r.astype(np.float64) - ro).max() < 2e-5
+
+
+def test_reconstruct_periodic(oracle):
+    _torch()
+    rng = np.random.default_rng(6)
+    pos = rng.random((20000, 3)) * 4.0
+    f = np.sin(np.pi * pos[:, 0] / 2) + np.cos(np.pi * pos[:, 2] / 2)
+    tp = rng.random((3000, 3)) * 4.0
+    M, O = _pair(oracle, pos, des_ngb=32, boxsize=4.0)
+    r = M.Reconstruct(f, tp, 1)
+    ro = O.Reconstruct(f, tp, 1)
+    assert np.array_equal(M.particle_mask, O.particle_mask)
+    assert _rel(r, ro).max() <= 1e-5
+
+
+def test_slice_and_interp_to_grid(oracle):
+    _torch()
+    rng = np.random.default_rng(7)
+    pos = rng.random((20000, 3))
+    f = np.sin(3 * pos[:, 0]) + pos[:, 1] * pos[:, 2]
+    c = np.array([0.5, 0.5, 0.5])
+    for plane in ("z", "x", "y"):
+        M, O = _pair(oracle, pos, des_ngb=32)
+        s = M.Slice(f, size=0.8, center=c, res=24, order=1, plane=plane)
+        so = O.Slice(f, 0.8, c, res=24, order=1, plane=plane)
+        assert s.shape == (24, 24) and _rel(s, so).max() <= 1e-5, plane
+    M, O = _pair(oracle, pos, des_ngb=32)
+    X, Y, Z, g = M.InterpToGrid(f, size=0.8, center=c, res=12, order=1, return_grid=True)
+    go = O.InterpToGrid(f, 0.8, c, res=12, order=1)
+    assert X.shape == (12, 12, 12) and g.shape == (12, 12, 12)
+    assert _rel(g, go).max() <= 1e-5
+    # defaults: center = median, size = 2 x the 90th-percentile radius
+    M = _pair(oracle, pos, des_ngb=32)[0]
+    x, y, s = M.Slice(f, res=16, return_grid=True)
+    assert s.shape == (16, 16) and np.isfinite(s).all()
