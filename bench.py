@@ -201,6 +201,73 @@ def batched_form(args, torch, dev, log):
                            "grad_median_rel": float(np.median(eg))}}
 
 
+def reconstruct_form(args, torch, dev, log, ix, nl_full, pos, pos_d, f, f_d, k, timed):
+    """Meshoid.InterpToGrid(f, order=1) on the headline particles after
+    Meshoid(pos): a res^3 grid of cell centres over [20, 80]^3; one step =
+    nearest particle of every grid point, np.unique of those, TreeUpdate of
+    the unique rows (rows of the unmasked list), D(f) on them and the
+    reconstruction (src/meshoid/Meshoid.py:383-440,504-555)."""
+    from meshoid_amd import engine
+    from oracle import oracle as orc
+
+    res = args.recon_res
+    e = np.linspace(20.0, 80.0, res + 1)
+    mid = 0.5 * (e[1:] + e[:-1])
+    X, Y, Z = np.meshgrid(mid, mid, mid, indexing="ij")
+    grid = np.column_stack((X.ravel(), Y.ravel(), Z.ravel()))
+    del X, Y, Z
+    tp_d = torch.from_numpy(grid).to(dev)
+    n = pos.shape[0]
+    nl2 = engine.NeighborList()
+    keep = {}
+
+    def step():
+        nbr = engine.nearest(ix, tp_d)
+        uniq, inv = engine.unique_inverse(nbr, n)
+        # TreeUpdate of the unique particles, from the unmasked list of the
+        # plain step (as Meshoid.TreeUpdate does after Meshoid(pos))
+        engine.nlist_subset(nl_full, uniq, nlist=nl2)
+        d1u, _, _ = engine.apply_D(nl2, f_d)
+        keep["r"] = engine.reconstruct(tp_d, pos_d, None, nbr, inv, f_d, d1u, None, 1)
+        keep["nbr"], keep["uniq"], keep["inv"], keep["d1u"] = nbr, uniq, inv, d1u
+
+    log(f"reconstruct: {res}^3 grid points")
+    dt, kern, stats = timed(step, max(args.steps // 2, 2), 1, 2 if args.stats else 1)
+    if stats.get("stat_nn_waves"):
+        w = stats["stat_nn_waves"]
+        log("nearest per-wave work:", {a[8:]: round(b / w, 2) for a, b in stats.items() if a.startswith("stat_nn")})
+    npt = grid.shape[0]
+    nu = int(keep["uniq"].numel())
+    # bytes per point: nearest = point 24 + index 8 + the particle's position 12;
+    # reconstruct = point 24 + nbr/inv 16 + position 12 + f 4 + D 12 + out 4
+    b_near, b_rec = 44.0, 72.0
+    t_near, t_rec = kern.get("nearest", float("nan")), kern.get("reconstruct", float("nan"))
+    # spot check: nearest exact, and f_target against the FP64 oracle's
+    # f(j) + dx . D(f)_j on a sample of grid points
+    rng = np.random.default_rng(args.seed + 11)
+    sel = np.sort(rng.choice(npt, 4096, replace=False))
+    p64 = pos.astype(np.float64)
+    near_o = orc.knn(p64, 1, queries=grid[sel])[1][:, 0]
+    nbr = keep["nbr"].cpu().numpy()[sel]
+    j = nbr
+    d, ngb = orc.knn(p64, k, queries=p64[j])
+    h_o = orc.hsml_iter(d, error_norm=1e-13)
+    G = orc.gradient_weights(p64, ngb, h_o, j)
+    g_o = orc.apply_weights(G, ngb, j, f.astype(np.float64)[:, None])[:, 0, :]
+    want = f.astype(np.float64)[j] + np.sum((grid[sel] - p64[j]) * g_o, axis=1)
+    got = keep["r"].cpu().numpy()[sel, 0].astype(np.float64)
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1.0)
+    return {"grid_points": npt, "unique_particles": nu, "order": 1, "ms_per_step": dt * 1e3,
+            "points_per_s": npt / dt,
+            "kernels_ms": {a: round(b, 4) for a, b in kern.items()},
+            "nearest_kernel": {"alg_bytes_per_point": b_near, "GB_s": npt * b_near / t_near / 1e6,
+                               "frac": npt * b_near / t_near / 1e-3 / HBM_PEAK},
+            "reconstruct_kernel": {"alg_bytes_per_point": b_rec, "GB_s": npt * b_rec / t_rec / 1e6,
+                                   "frac": npt * b_rec / t_rec / 1e-3 / HBM_PEAK},
+            "spot_check": {"points": 4096, "nearest_equal": int((nbr == near_o).sum()),
+                           "f_max_rel": float(err.max()), "f_median_rel": float(np.median(err))}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +283,8 @@ def main():
     ap.add_argument("--plain-only", action="store_true", help="skip the learned-seed form")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched form")
     ap.add_argument("--batch-snapshots", type=int, default=64)
+    ap.add_argument("--no-recon", action="store_true", help="skip the InterpToGrid measurement")
+    ap.add_argument("--recon-res", type=int, default=128)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a known-size kernel "
                          "that scales the profiler's HBM byte counters)")
@@ -361,6 +430,9 @@ def main():
                        fields2=F2, d1o2=o1.cpu().numpy().astype(np.float64),
                        d2=o2.cpu().numpy().astype(np.float64))
         precision["errors_vs_fp64_oracle"] = acc.pop("order2")
+    recon = None
+    if not args.no_recon:
+        recon = reconstruct_form(args, torch, dev, log, ix, nl, pos, pos_d, f, f_d, k, timed)
     batched = None
     if not args.no_batched:
         batched = batched_form(args, torch, dev, log)
@@ -396,6 +468,7 @@ def main():
         "accuracy": acc,
         "precision": precision,
         "batched": batched,
+        "reconstruct": recon,
         "cpu_baseline": cpu,
         "library": meshoid_amd._lib.LIB_PATH,
     }

@@ -201,6 +201,18 @@ int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int
 int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
 
 /*
+ * TreeUpdate with a particle_mask (src/meshoid/Meshoid.py:147-170) served
+ * from an unmasked list `full` of the same index and des_ngb: a particle's
+ * neighbours, distances and smoothing length do not depend on the mask, so
+ * the rows qidx int64[nq] of `full` are copied into `nl` (a list other than
+ * `full`) and density/volume recomputed with m_rows float64[nq] (NULL = 1),
+ * as mhip_knn_hsml with qidx would give them.  h_out/rho_out/vol_out float32
+ * and status_out int32 [nq] in row order, each may be NULL.
+ */
+int mhip_nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int64_t nq, const double *m_rows,
+                      float *h_out, float *rho_out, float *vol_out, int32_t *status_out, mhip_nlist *nl);
+
+/*
  * Meshoid.Reconstruct (src/meshoid/Meshoid.py:383-440), in three steps.
  *
  * mhip_nearest: the nearest particle of each point, tree.query(points)[1]

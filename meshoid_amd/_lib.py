@@ -29,7 +29,7 @@ EXPORTED = [
     "mhip_seed_predict", "mhip_mlp_train_step", "mhip_mlp_pack", "mhip_mlp_blob_bytes",
     "mhip_kernel_weights", "mhip_kernel_average", "mhip_kernel_variance", "mhip_integrate",
     "mhip_batch_create", "mhip_batch_run", "mhip_batch_info", "mhip_batch_destroy",
-    "mhip_index_order", "mhip_nearest", "mhip_unique_inverse", "mhip_reconstruct",
+    "mhip_index_order", "mhip_nearest", "mhip_unique_inverse", "mhip_reconstruct", "mhip_nlist_subset",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -82,6 +82,7 @@ _SIGS = {
     "mhip_batch_destroy": ([_p], _i),
     "mhip_index_order": ([_p, _p, _p], _i),
     "mhip_nearest": ([_p, _p, _p, _i64, _i, _p], _i),
+    "mhip_nlist_subset": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _p], _i),
     "mhip_unique_inverse": ([_p, _p, _i64, _i64, _p, _p, _p], _i),
     "mhip_reconstruct": ([_p, _p, _i64, _i, _d, _p, _i, _p, _p, _p, _i, _i, _p, _p, _i, _p], _i),
 }

@@ -133,7 +133,8 @@ int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen) {
         static const char *stat_names[STAT_COUNT] = {"stat_p1_leaves", "stat_p1_inserts", "stat_p1_nodes",
                                                      "stat_p2_leaves", "stat_p2_nodes", "stat_waves",
                                                      "stat_lane_need", "stat_maxlane_need",
-                                                     "stat_parent_max"};
+                                                     "stat_parent_max", "stat_nn_waves", "stat_nn_nodes",
+                                                     "stat_nn_leaves", "stat_nn_exact"};
         unsigned long long hs[16];
         rc = check_hip(ctx, hipMemcpy(hs, ctx->stats, sizeof hs, hipMemcpyDeviceToHost), "stats read");
         if (rc) return rc;
@@ -336,6 +337,14 @@ int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, in
     MHIP_NEED(vol && f && out, "null argument");
     hipSetDevice(ctx->device);
     return integrate(ctx, vol, f, f_f64 ? 1 : 0, n, K, out);
+}
+
+int mhip_nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int64_t nq, const double *m_rows,
+                      float *h_out, float *rho_out, float *vol_out, int32_t *status_out, mhip_nlist *nl) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(full && qidx && nl, "null argument");
+    hipSetDevice(ctx->device);
+    return nlist_subset(ctx, full, qidx, nq, m_rows, h_out, rho_out, vol_out, status_out, nl);
 }
 
 int mhip_nearest(mhip_ctx *ctx, const mhip_index *ix, const double *points, int64_t nt, int dim,

@@ -134,6 +134,10 @@ enum : int {
     STAT_LANE_NEED,       // leaves under a leaf-parent that a lane itself needs (sum over lanes)
     STAT_MAXLANE_NEED,    // per wavefront, the largest such count of one lane (sum over waves)
     STAT_PARENT_MAX,      // per leaf-parent expansion, the largest count of one lane (sum)
+    STAT_NN_WAVES,        // wavefronts of the nearest-point query
+    STAT_NN_NODES,        // its wave-uniform node tests (sum over waves)
+    STAT_NN_LEAVES,       // its wave-uniform leaf scans (sum over waves)
+    STAT_NN_EXACT,        // points re-ranked in float64
     STAT_COUNT
 };
 static_assert(STAT_COUNT <= 16, "the work-counter buffer holds 16 entries");

@@ -93,16 +93,6 @@ __global__ __launch_bounds__(256) void k_bbox_final(const float *__restrict__ pa
 // ---------------------------------------------------------------------------
 // Morton keys (21 bits per axis, x in bit 0)
 // ---------------------------------------------------------------------------
-MHIP_DEV uint64_t spread21(uint64_t x) {
-    x &= 0x1fffffull;
-    x = (x | x << 32) & 0x1f00000000ffffull;
-    x = (x | x << 16) & 0x1f0000ff0000ffull;
-    x = (x | x << 8) & 0x100f00f00f00f00full;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
-    x = (x | x << 2) & 0x1249249249249249ull;
-    return x;
-}
-
 template <typename PT>
 __global__ __launch_bounds__(256) void k_morton(const PT *__restrict__ pos, int64_t n, int dim,
                                                const float *__restrict__ bb,

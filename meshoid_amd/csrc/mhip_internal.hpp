@@ -10,6 +10,18 @@ int fail(mhip_ctx *ctx, int code, const std::string &msg);
 int check_hip(mhip_ctx *ctx, hipError_t e, const char *what);
 void *scratch(mhip_ctx *ctx, size_t bytes, int *rc);
 
+// the 21 low bits of x spread to every third bit (one axis of a 63-bit
+// Morton key)
+MHIP_DEV inline uint64_t spread21(uint64_t x) {
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
 // LSD radix sort of (u64 key, u32 value) pairs on bits [0, bits) of the key.
 // Buffers: k0/v0 hold the input; the sorted result is returned in *kout/*vout
 // (one of the two ping-pong buffers).  counts needs 256*ceil(n/4096)+256 u32.
@@ -39,6 +51,8 @@ int mlp_pack(mhip_ctx *ctx, const float *P, const float *mu, const float *sd, vo
 size_t mlp_blob_bytes();
 
 int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out);
+int nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int64_t nq, const double *m_rows,
+                 float *h_out, float *rho_out, float *vol_out, int32_t *status_out, mhip_nlist *nl);
 
 // nearest particle of arbitrary points (mhip_knn.hip) and the reconstruction
 // around it (mhip_recon.hip)

@@ -904,6 +904,44 @@ static void launch_main(const KnnArgs &a, bool per, bool p64, dim3 g, hipStream_
     }
 }
 
+static int nl_reserve(mhip_ctx *ctx, mhip_nlist *nl, int64_t nq, int k) {
+    int rc = 0;
+    if (nl->cap < nq || nl->k < k) {
+        void **arr[] = {(void **)&nl->qsorted, (void **)&nl->qrow, (void **)&nl->tau,
+                        (void **)&nl->nlt,     (void **)&nl->h,    (void **)&nl->status};
+        for (auto p : arr)
+            if ((rc = nl_grow(ctx, p, nq * 4))) return rc;
+        if ((rc = nl_grow(ctx, (void **)&nl->amb, (nq + 1) * 4))) return rc;
+        if ((rc = nl_grow(ctx, (void **)&nl->idx, (size_t)nq * k * 4))) return rc;
+        if ((rc = nl_grow(ctx, (void **)&nl->d2, (size_t)nq * k * 4))) return rc;
+        nl->cap = nq;
+        nl->k = k;
+    }
+    return 0;
+}
+
+// slots of the query rows qidx in sorted (Morton) order: qsorted[s] is the
+// sorted-space particle of slot s, qrow[s] its row in qidx
+static int query_setup(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, mhip_nlist *nl) {
+    ProfRegion pr(ctx, "knn_query_setup");
+    int rc = 0;
+    size_t cw = radix_counts_words(nq);
+    char *s = (char *)scratch(ctx, nq * 24 + cw * 4, &rc);
+    if (!s) return rc;
+    uint64_t *k0 = (uint64_t *)s, *k1 = k0 + nq;
+    uint32_t *v0 = (uint32_t *)(k1 + nq), *v1 = v0 + nq, *counts = v1 + nq;
+    hipLaunchKernelGGL(k_qsetup_mask, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream, qidx, nq,
+                       ix->iperm, k0, v0);
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < ix->n) bits++;
+    uint64_t *ks;
+    uint32_t *vs;
+    if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nq, bits, &ks, &vs))) return rc;
+    hipLaunchKernelGGL(k_qsetup_finish, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream, ks, vs,
+                       nq, nl->qsorted, nl->qrow);
+    return 0;
+}
+
 static TreeView tree_view(const mhip_index *ix) {
     TreeView t;
     t.sx = ix->sx;
@@ -934,17 +972,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     if (!qidx) nq = ix->n;
     if (nq < 1) return fail(ctx, -1, "no query rows");
     int rc = 0;
-    if (nl->cap < nq || nl->k < k) {
-        void **arr[] = {(void **)&nl->qsorted, (void **)&nl->qrow, (void **)&nl->tau,
-                        (void **)&nl->nlt,     (void **)&nl->h,    (void **)&nl->status};
-        for (auto p : arr)
-            if ((rc = nl_grow(ctx, p, nq * 4))) return rc;
-        if ((rc = nl_grow(ctx, (void **)&nl->amb, (nq + 1) * 4))) return rc;
-        if ((rc = nl_grow(ctx, (void **)&nl->idx, (size_t)nq * k * 4))) return rc;
-        if ((rc = nl_grow(ctx, (void **)&nl->d2, (size_t)nq * k * 4))) return rc;
-        nl->cap = nq;
-        nl->k = k;
-    }
+    if ((rc = nl_reserve(ctx, nl, nq, k))) return rc;
     nl->nq = nq;
     nl->k = k;
     nl->index = ix;
@@ -952,23 +980,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     // query slots in Morton order
     const bool masked = qidx != nullptr;
     nl->masked = masked;
-    if (masked) {
-        ProfRegion pr(ctx, "knn_query_setup");
-        size_t cw = radix_counts_words(nq);
-        char *s = (char *)scratch(ctx, nq * 24 + cw * 4, &rc);
-        if (!s) return rc;
-        uint64_t *k0 = (uint64_t *)s, *k1 = k0 + nq;
-        uint32_t *v0 = (uint32_t *)(k1 + nq), *v1 = v0 + nq, *counts = v1 + nq;
-        hipLaunchKernelGGL(k_qsetup_mask, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
-                           ctx->stream, qidx, nq, ix->iperm, k0, v0);
-        int bits = 1;
-        while (bits < 32 && ((int64_t)1 << bits) < ix->n) bits++;
-        uint64_t *ks;
-        uint32_t *vs;
-        if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nq, bits, &ks, &vs))) return rc;
-        hipLaunchKernelGGL(k_qsetup_finish, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
-                           ctx->stream, ks, vs, nq, nl->qsorted, nl->qrow);
-    }
+    if (masked && (rc = query_setup(ctx, ix, qidx, nq, nl))) return rc;
     KnnArgs a;
     a.t = tree_view(ix);
     const bool per = ix->box > 0;
@@ -1102,6 +1114,68 @@ __global__ __launch_bounds__(256) void k_export(const uint32_t *__restrict__ idx
     }
 }
 
+// ---------------------------------------------------------------------------
+// rows of a full-set list for a subset of query rows.  A particle's k nearest
+// neighbours, their distances and its smoothing length do not depend on
+// which other particles are queried, so TreeUpdate with a particle_mask
+// (src/meshoid/Meshoid.py:147-170) equals these rows of the unmasked list;
+// density and volume use the subset's masses (Meshoid.py:169-170).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nl_subset(
+    const uint32_t *__restrict__ fidx, const float *__restrict__ fd2, const float *__restrict__ ftau,
+    const int32_t *__restrict__ fnlt, const float *__restrict__ fh, const int32_t *__restrict__ fst,
+    int64_t nfull, const uint32_t *__restrict__ qsorted, const uint32_t *__restrict__ qrow, int64_t nq, int k,
+    int dim, const double *__restrict__ m_rows, uint32_t *__restrict__ idx, float *__restrict__ d2,
+    float *__restrict__ tau, int32_t *__restrict__ nlt, float *__restrict__ hs, int32_t *__restrict__ st,
+    float *__restrict__ h_out, float *__restrict__ rho_out, float *__restrict__ vol_out,
+    int32_t *__restrict__ status_out) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nq) return;
+    const uint32_t f = qsorted[s], row = qrow[s];
+    for (int j = 0; j < k; j++) {
+        idx[(int64_t)j * nq + s] = fidx[(int64_t)j * nfull + f];
+        if (d2) d2[(int64_t)j * nq + s] = fd2[(int64_t)j * nfull + f];
+    }
+    const float h = fh[f];
+    tau[s] = ftau[f];
+    nlt[s] = fnlt[f];
+    hs[s] = h;
+    st[s] = fst[f];
+    // the main kernel's density and volume arithmetic
+    const double vn = dim == 3 ? 4.18879020478639098 : (dim == 2 ? 3.14159265358979324 : 2.0);
+    const double m = m_rows ? m_rows[row] : 1.0;
+    const double hd = (double)h, hp = dim == 3 ? hd * hd * hd : (dim == 2 ? hd * hd : hd);
+    const double rho = (double)k * m / (vn * hp);
+    if (h_out) h_out[row] = h;
+    if (rho_out) rho_out[row] = (float)rho;
+    if (vol_out) vol_out[row] = (float)(m / rho);
+    if (status_out) status_out[row] = fst[f];
+}
+
+int nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int64_t nq, const double *m_rows,
+                 float *h_out, float *rho_out, float *vol_out, int32_t *status_out, mhip_nlist *nl) {
+    const mhip_index *ix = full->index;
+    if (!ix || full->nq < 1) return fail(ctx, -1, "neighbour list is empty");
+    if (full->masked || full->nq != ix->n) return fail(ctx, -1, "the source list must cover every particle");
+    if (nl == full) return fail(ctx, -1, "the subset needs a list of its own");
+    if (nq < 1) return fail(ctx, -1, "no query rows");
+    const int k = full->k;
+    int rc = 0;
+    if ((rc = nl_reserve(ctx, nl, nq, k))) return rc;
+    nl->nq = nq;
+    nl->k = k;
+    nl->index = ix;
+    nl->has_d2 = full->has_d2;
+    nl->masked = true;
+    if ((rc = query_setup(ctx, ix, qidx, nq, nl))) return rc;
+    ProfRegion pr(ctx, "nlist_subset");
+    hipLaunchKernelGGL(k_nl_subset, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream, full->idx,
+                       full->has_d2 ? full->d2 : nullptr, full->tau, full->nlt, full->h, full->status, full->nq,
+                       nl->qsorted, nl->qrow, nq, k, ix->dim, m_rows, nl->idx, full->has_d2 ? nl->d2 : nullptr,
+                       nl->tau, nl->nlt, nl->h, nl->status, h_out, rho_out, vol_out, status_out);
+    return check_hip(ctx, hipGetLastError(), "subset launch");
+}
+
 int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out) {
     const mhip_index *ix = nl->index;
     if (!ix || nl->nq < 1) return fail(ctx, -1, "neighbour list is empty");
@@ -1115,26 +1189,35 @@ int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *d
 
 // ---------------------------------------------------------------------------
 // nearest particle of arbitrary float64 points: cKDTree.query(x)[1], k = 1
-// (src/meshoid/Meshoid.py:409).  One thread per point: (1) a greedy descent
-// to the leaf whose box is nearest gives a first bound; (2) a depth-first
-// walk over every node within that bound (widened by the float32 rounding
-// band) keeps the nearest and the second-nearest float32 d2; (3) when those
-// two lie closer than float32 rounding can order, the band is re-ranked with
-// float64 distances from the caller's positions (the distance cKDTree
-// computes), ties to the lowest original index.  The point is split into
-// float32 hi + lo parts, so its float32 distances are as accurate as those
-// between two particles.
+// (src/meshoid/Meshoid.py:409).  The points are first put in the index's
+// Morton order, so that a wavefront's 64 lanes serve a compact block of
+// points.  (1) Each lane finds, by binary search over the leaves' first
+// Morton keys, the leaf where its point's key falls, and scans it and its
+// two neighbours (in Morton order these usually hold the nearest particle):
+// every lane starts with a tight bound.  (2) A wave-uniform walk, inside out
+// from the middle lane's leaf (its siblings, then the rest of the
+// grandparent's subtree, ... up to the root), tests each node once for all
+// lanes and scans the leaves some lane's bound (widened by the float32
+// rounding band) reaches; it keeps the nearest and the second-nearest
+// float32 d2.  (3) Where those two lie closer than float32 rounding can
+// order, the band is re-ranked with float64 distances from the caller's
+// positions (the distance cKDTree computes), ties to the lowest original
+// index.  The point is split into float32 hi + lo parts, so its float32
+// distances are as accurate as those between two particles.
 // ---------------------------------------------------------------------------
 struct NearestArgs {
     TreeView t;
+    const uint32_t *order;  // point of each thread (Morton order), null = identity
     const uint32_t *perm;
     const void *pos_in;
     int64_t n;
     int dim;
     double box;        // > 0: periodic in [0, box)
+    float org[3], scale;  // the index's Morton quantisation frame
     const double *tg;  // (nt, dim) row-major
     int64_t nt;
     int64_t *out;      // original particle index per point, -1 for a non-finite point
+    unsigned long long *stats;  // work counters, may be null
 };
 
 template <typename PT, bool PER>
@@ -1148,29 +1231,48 @@ MHIP_DEV double exact_d2_point(const PT *pos, int dim, uint32_t j, const double 
     return s;
 }
 
-template <bool PER, bool P64>
-MHIP_DEV void nearest_scan(const TreeView &t, const QPos &q, int64_t leaf, float &best, float &second,
-                           uint32_t &bp) {
-    float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
-    load_leaf<P64>(t, leaf, cx, cy, cz, clx, cly, clz);
-#pragma unroll
-    for (int j = 0; j < LEAF; j++) {
-        // padding particles carry NaN coordinates: never below either bound
-        const float d2 = dist2<PER, true>(t, q, cx[j], cy[j], cz[j], clx[j], cly[j], clz[j]);
-        if (d2 < best) {
-            second = best;
-            best = d2;
-            bp = (uint32_t)(leaf * LEAF + j);
-        } else if (d2 < second) {
-            second = d2;
-        }
+// Morton key of each point in the index's quantisation frame (as the
+// particles' keys are formed), periodic points wrapped into the box first
+__global__ __launch_bounds__(256) void k_point_keys(const double *__restrict__ tg, int64_t nt, int dim,
+                                                    double box, float ox, float oy, float oz, float scale,
+                                                    uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const float org[3] = {ox, oy, oz};
+    uint64_t key = 0;
+    for (int c = 0; c < dim; c++) {
+        double v = tg[i * dim + c];
+        if (box > 0.0 && (v < 0.0 || v >= box)) v -= floor(v / box) * box;
+        float t = ((float)v - org[c]) * scale;
+        t = fminf(fmaxf(t, 0.f), 2097151.0f);  // NaN -> 0
+        key |= spread21((uint64_t)t) << c;
     }
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+}
+
+// Morton key of sorted particle p, formed as the index build forms it
+MHIP_DEV uint64_t particle_key(const TreeView &t, const float (&org)[3], float scale, int dim, int64_t p) {
+    const float v[3] = {t.sx[p], t.sy[p], t.sz[p]};
+    uint64_t key = 0;
+    for (int c = 0; c < dim; c++) {
+        float u = (v[c] - org[c]) * scale;
+        u = fminf(fmaxf(u, 0.f), 2097151.0f);
+        key |= spread21((uint64_t)u) << c;
+    }
+    return key;
 }
 
 template <bool PER, bool P64, typename PT>
 __global__ __launch_bounds__(256) void k_nearest(NearestArgs a) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= a.nt) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t wave0 = tid - lane;
+    if (wave0 >= a.nt) return;
+    const bool live = tid < a.nt;
+    // lanes past the end take the wave's first point (never written)
+    const int64_t slot = live ? tid : wave0;
+    const int64_t i = a.order ? (int64_t)a.order[slot] : slot;
     double tp[3] = {0.0, 0.0, 0.0};
     for (int c = 0; c < a.dim; c++) {
         double v = a.tg[i * a.dim + c];
@@ -1187,62 +1289,132 @@ __global__ __launch_bounds__(256) void k_nearest(NearestArgs a) {
     q.lx = (float)(tp[0] - (double)q.x);
     q.ly = (float)(tp[1] - (double)q.y);
     q.lz = (float)(tp[2] - (double)q.z);
-    // box tests use the hi parts: the slack covers the point's lo parts too
+    // box tests use the hi parts: the slack covers the point's lo parts and
+    // (float64 particles) the particles' lo parts
     TreeView t = a.t;
     t.slack = a.t.slack + fmaxf(fabsf(q.lx), fmaxf(fabsf(q.ly), fabsf(q.lz)));
+    QBox qb;
+    qb.lx = q.x - t.slack;
+    qb.ly = q.y - t.slack;
+    qb.lz = q.z - t.slack;
+    qb.hx = q.x + t.slack;
+    qb.hy = q.y + t.slack;
+    qb.hz = q.z + t.slack;
     const uint32_t first_leaf = (uint32_t)t.first_leaf;
-    // (1) greedy descent (a non-empty node always has a nearest non-empty child)
-    uint32_t x = 0;
-    while (x < first_leaf) {
-        const uint32_t c0 = FANOUT * x + 1;
-        uint32_t bc = c0;
-        float bd = node_d2<PER>(t, q, c0);
-        for (uint32_t c = 1; c < FANOUT; c++) {
-            const float d = node_d2<PER>(t, q, c0 + c);
-            if (d < bd) {
-                bd = d;
-                bc = c0 + c;
-            }
+    // (1) this lane's leaf: the last leaf whose first key is at most the
+    // point's key; it and its neighbours give the lane a first bound
+    uint64_t kq = 0;
+    {
+        const double v[3] = {tp[0], tp[1], tp[2]};
+        for (int c = 0; c < a.dim; c++) {
+            float u = ((float)v[c] - a.org[c]) * a.scale;
+            u = fminf(fmaxf(u, 0.f), 2097151.0f);  // NaN -> 0
+            kq |= spread21((uint64_t)u) << c;
         }
-        x = bc;
     }
-    const uint32_t gl = x;
+    uint32_t lo = 0, hi = (uint32_t)t.nleaf;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (particle_key(t, a.org, a.scale, a.dim, (int64_t)mid * LEAF) <= kq)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t own_lo = lo > 0 ? lo - 1 : 0, own_hi = min(lo + 1, (uint32_t)t.nleaf - 1);
     float best = INFINITY, second = INFINITY;
     uint32_t bp = 0xffffffffu;
-    if (gl - first_leaf < t.nleaf) nearest_scan<PER, P64>(t, q, gl - first_leaf, best, second, bp);
-    // (2) every other leaf within the bound
-    x = 0;
-    bool more = true;
-    while (more) {
-        if (node_d2<PER>(t, q, x) <= best * ((1.0f + BAND_REL) * PRUNE_REL)) {
-            if (x < first_leaf) {
-                x = FANOUT * x + 1;
-                continue;
-            }
-            if (x != gl && x - first_leaf < t.nleaf) nearest_scan<PER, P64>(t, q, x - first_leaf, best, second, bp);
+    float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
+    // candidates of one leaf; skip: this lane has the leaf already
+    auto take = [&](int64_t leaf, bool skip) {
+#pragma unroll
+        for (int j = 0; j < LEAF; j++) {
+            // padding particles carry NaN coordinates: never below a bound
+            const float d2 = dist2<PER, true>(t, q, cx[j], cy[j], cz[j], clx[j], cly[j], clz[j]);
+            const bool nb = !skip && d2 < best;
+            second = nb ? best : (skip ? second : fminf(second, d2));
+            best = nb ? d2 : best;
+            bp = nb ? (uint32_t)(leaf * LEAF + j) : bp;
         }
-        more = next_after(x);
+    };
+    auto scan = [&](uint32_t node) {
+        const int64_t leaf = node - first_leaf;
+        load_leaf_uniform<P64>(t.sx, t.sy, t.sz, t.lx, t.ly, t.lz, leaf, cx, cy, cz, clx, cly, clz);
+        take(leaf, leaf >= own_lo && leaf <= own_hi);
+    };
+    for (uint32_t l = own_lo; l <= own_hi; l++) {
+        load_leaf<P64>(t, l, cx, cy, cz, clx, cly, clz);
+        take(l, false);
     }
+    // the middle lane's leaf, for every lane
+    const uint32_t x = first_leaf + uniform(__shfl(lo, WAVE / 2));
+    uint32_t n_nodes = 0, n_leaves = 1;
+    scan(x);
+    // (2) wave-uniform walk, inside out: the subtree of each ancestor of the
+    // middle lane's leaf in turn (parent, grandparent, ... root), minus the
+    // subtree already done, depth first and pruned by every lane's bound.
+    const float F = (1.0f + BAND_REL) * PRUNE_REL;
+    uint32_t done = x;  // scanned
+    while (done != 0) {
+        const uint32_t top = (done - 1) >> 3;
+        uint32_t y = FANOUT * top + 1;
+        while (true) {
+            if (y != done) {
+                n_nodes++;
+                if (__any(box_gap2_u<PER>(t, qb, y) <= best * F)) {
+                    if (y < first_leaf) {
+                        y = FANOUT * y + 1;
+                        continue;
+                    }
+                    if (y - first_leaf < t.nleaf) {
+                        scan(y);
+                        n_leaves++;
+                    }
+                }
+            }
+            // next node after the subtree of y, within the subtree of top
+            bool fin = false;
+            while ((y & (FANOUT - 1)) == 0) {  // y is the last child of its parent
+                y = (y - 1) >> 3;
+                if (y == top) {
+                    fin = true;
+                    break;
+                }
+            }
+            if (fin) break;
+            y++;
+        }
+        done = top;
+    }
+    if (a.stats) {
+        const bool ex = live && bp != 0xffffffffu && second <= best * (1.0f + AMBIG_REL);
+        const uint32_t nex = (uint32_t)__popcll(__ballot(ex));
+        if (lane == 0) {
+            atomicAdd(a.stats + STAT_NN_WAVES, 1ull);
+            atomicAdd(a.stats + STAT_NN_NODES, (unsigned long long)n_nodes);
+            atomicAdd(a.stats + STAT_NN_LEAVES, (unsigned long long)n_leaves);
+            atomicAdd(a.stats + STAT_NN_EXACT, (unsigned long long)nex);
+        }
+    }
+    if (!live) return;
     int64_t res = -1;
     if (bp != 0xffffffffu) {
         res = a.perm[bp];
         if (second <= best * (1.0f + AMBIG_REL)) {
-            // (3) float64 re-rank of the rounding band
+            // (3) float64 re-rank of this point's rounding band (rare; per lane)
             const PT *pos = (const PT *)a.pos_in;
             const float lim = best * (1.0f + BAND_REL);
             double be = INFINITY;
             uint32_t bo = 0xffffffffu;
-            x = 0;
-            more = true;
+            uint32_t y = 0;
+            bool more = true;
             while (more) {
-                if (node_d2<PER>(t, q, x) <= lim * PRUNE_REL) {
-                    if (x < first_leaf) {
-                        x = FANOUT * x + 1;
+                if (node_d2<PER>(t, q, y) <= lim * PRUNE_REL) {
+                    if (y < first_leaf) {
+                        y = FANOUT * y + 1;
                         continue;
                     }
-                    if (x - first_leaf < t.nleaf) {
-                        const int64_t leaf = x - first_leaf;
-                        float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
+                    if (y - first_leaf < t.nleaf) {
+                        const int64_t leaf = y - first_leaf;
                         load_leaf<P64>(t, leaf, cx, cy, cz, clx, cly, clz);
                         for (int j = 0; j < LEAF; j++) {
                             const int64_t p = leaf * LEAF + j;
@@ -1258,7 +1430,7 @@ __global__ __launch_bounds__(256) void k_nearest(NearestArgs a) {
                         }
                     }
                 }
-                more = next_after(x);
+                more = next_after(y);
             }
             res = bo;
         }
@@ -1274,17 +1446,37 @@ int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int6
     if (nt == 0) return 0;
     NearestArgs a;
     a.t = tree_view(ix);
+    a.order = nullptr;
     a.perm = ix->perm;
     a.pos_in = ix->pos_in;
     a.n = ix->n;
     a.dim = ix->dim;
     a.box = ix->box > 0 ? ix->box : 0.0;
+    for (int c = 0; c < 3; c++) a.org[c] = ix->frame[c];
+    a.scale = ix->frame[3];
     a.tg = targets;
     a.nt = nt;
     a.out = out;
+    a.stats = ctx->prof >= 2 ? ctx->stats : nullptr;
     const bool per = ix->box > 0, p64 = ix->pos_f64 != 0;
     const dim3 g((unsigned)((nt + 255) / 256));
     hipStream_t s = ctx->stream;
+    if (nt >= 4096 && nt <= 0xffffffffll) {
+        // points in the index's Morton order
+        ProfRegion pk(ctx, "nearest_sort");
+        int rc = 0;
+        const size_t cw = radix_counts_words(nt);
+        char *sc = (char *)scratch(ctx, (size_t)nt * 24 + cw * 4, &rc);
+        if (!sc) return rc;
+        uint64_t *k0 = (uint64_t *)sc, *k1 = k0 + nt;
+        uint32_t *v0 = (uint32_t *)(k1 + nt), *v1 = v0 + nt, *counts = v1 + nt;
+        hipLaunchKernelGGL(k_point_keys, g, dim3(256), 0, s, targets, nt, dim, a.box, ix->frame[0], ix->frame[1],
+                           ix->frame[2], ix->frame[3], k0, v0);
+        uint64_t *ks;
+        uint32_t *vs;
+        if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nt, 21 * dim, &ks, &vs))) return rc;
+        a.order = vs;
+    }
     ProfRegion pr(ctx, "nearest");
     if (per) {
         if (p64)

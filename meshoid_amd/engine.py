@@ -173,6 +173,30 @@ def knn_hsml(index: SpatialIndex, k: int, qidx=None, m_rows=None, seed_h=None, k
     return out
 
 
+def nlist_subset(full: NeighborList, qidx: torch.Tensor, m_rows=None, nlist: NeighborList | None = None):
+    """The rows qidx of an unmasked list: what knn_hsml(index, k, qidx=qidx,
+    m_rows=m_rows) returns, without a new search (a particle's neighbours and
+    smoothing length do not depend on the mask)."""
+    _need(qidx, (torch.int64,), "qidx")
+    _need(m_rows, (torch.float64,), "m_rows")
+    index = full.index
+    dev = index.pos.device
+    nq = qidx.numel()
+    if nq and (int(qidx.min()) < 0 or int(qidx.max()) >= index.n):
+        raise IndexError("query index out of range")
+    if m_rows is not None and m_rows.numel() != nq:
+        raise ValueError("m_rows must have one entry per query row")
+    out = {name: torch.empty(nq, dtype=torch.float32, device=dev) for name in ("h", "rho", "vol")}
+    out["status"] = torch.empty(nq, dtype=torch.int32, device=dev)
+    nl = nlist if nlist is not None else NeighborList()
+    ctx = context(dev)
+    raise_for(lib.mhip_nlist_subset(ctx, full.h, ptr(qidx), nq, ptr(m_rows), ptr(out["h"]), ptr(out["rho"]),
+                                    ptr(out["vol"]), ptr(out["status"]), nl.h), ctx)
+    nl.nq, nl.k, nl.index, nl.keep_dist = nq, full.k, index, full.keep_dist
+    out["nlist"] = nl
+    return out
+
+
 def apply_D(nl: NeighborList, f: torch.Tensor, order=1, weighted=True, want_d1=True,
             want_d2=False, out1=None, out2=None, status=None):
     """Least-squares derivatives of f over the list's rows.

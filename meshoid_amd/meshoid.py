@@ -71,6 +71,7 @@ class Meshoid:
         self._identity_mask = self.Nmask == self.N and np.array_equal(
             self.particle_mask, np.arange(self.N))
         self._nl = None
+        self._nl_full = None  # the unmasked list of the current tree, kept for masked updates
         self._dev = {}
         self._np = {}
         self._dw = {}
@@ -180,6 +181,7 @@ class Meshoid:
         if self.verbose:
             print("Building tree...")
         self.tree = engine.SpatialIndex().build(self._pos_d, self.boxsize)
+        self._nl_full = None
 
     def TreeUpdate(self):
         """Neighbour lists, smoothing lengths and densities (Meshoid.py:147-170)."""
@@ -196,8 +198,19 @@ class Meshoid:
             if pm.size and (pm.min() < -self.N or pm.max() >= self.N):
                 raise IndexError("particle_mask index out of range")
             qidx = torch.from_numpy(np.where(pm < 0, pm + self.N, pm)).to(dev)
-        out = engine.knn_hsml(self.tree, self.des_ngb, qidx=qidx, m_rows=m_d, keep_dist=True,
-                              nlist=self._nl)
+        full = self._nl_full
+        if qidx is not None and full is not None and full.k == self.des_ngb and full.index is self.tree:
+            # the rows of the unmasked list are the masked search's result
+            if self._nl is full:
+                self._nl = None
+            out = engine.nlist_subset(full, qidx, m_rows=m_d, nlist=self._nl)
+        else:
+            if qidx is not None and self._nl is full:
+                self._nl = None  # keep the unmasked list intact
+            out = engine.knn_hsml(self.tree, self.des_ngb, qidx=qidx, m_rows=m_d, keep_dist=True,
+                                  nlist=self._nl)
+            if qidx is None:
+                self._nl_full = out["nlist"]
         if self.verbose:
             print("Neighbours found!")
             print("Iterating for smoothing lengths...")

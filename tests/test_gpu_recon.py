@@ -190,3 +190,32 @@ def test_slice_and_interp_to_grid(oracle):
     M = _pair(oracle, pos, des_ngb=32)[0]
     x, y, s = M.Slice(f, res=16, return_grid=True)
     assert s.shape == (16, 16) and np.isfinite(s).all()
+
+
+def test_masked_update_from_full_list_equals_search():
+    """TreeUpdate with a mask served from the unmasked list (a particle's
+    neighbours and h do not depend on the mask) equals the masked search bit
+    for bit: lists, distances, h, rho (with the subset's masses), status."""
+    torch = _torch()
+    from meshoid_amd import engine, synthetic
+
+    pos = synthetic.clustered(60000, seed=8, nclump=12, box=10.0, background=0.1)
+    pd = torch.from_numpy(pos).cuda()
+    ix = engine.SpatialIndex().build(pd)
+    full = engine.knn_hsml(ix, 32, keep_dist=True)["nlist"]
+    rng = np.random.default_rng(9)
+    q = torch.from_numpy(np.sort(rng.choice(60000, 7000, replace=False))[::-1].copy()).cuda()
+    m = torch.from_numpy(rng.random(7000) + 0.5).cuda()
+    a = engine.knn_hsml(ix, 32, qidx=q, m_rows=m, keep_dist=True)
+    b = engine.nlist_subset(full, q, m_rows=m)
+    for key in ("h", "rho", "vol", "status"):
+        assert torch.equal(a[key], b[key]), key
+    na, da = a["nlist"].export()
+    nb, db = b["nlist"].export()
+    assert torch.equal(na, nb) and torch.equal(da, db)
+    f = torch.from_numpy(rng.standard_normal((60000, 3))).cuda()
+    ga = engine.apply_D(a["nlist"], f)[0]
+    gb = engine.apply_D(b["nlist"], f)[0]
+    assert torch.equal(ga, gb)
+    with pytest.raises(ValueError):
+        engine.nlist_subset(a["nlist"], q)  # the source must be unmasked
