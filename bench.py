@@ -381,7 +381,17 @@ def main():
         plain_h = engine.knn_hsml(ix, k, keep_dist=False)["h"]
         lerr = (torch.log(hp) - torch.log(plain_h)).abs()
         outl = engine.knn_hsml(ix, k, keep_dist=False, seed_h=hp, seed_r=(beta * hp).contiguous())
+        # Newton iterations of the smoothing-length solve, plain vs seeded
+        # (kNN work counters of one extra step each)
+        newton = {}
+        for nm, fn in (("plain", step_plain), ("learned", lambda: step_learned(beta))):
+            st_ = timed(fn, 1, 0, 2)[2]
+            w = max(st_.get("stat_waves", 1), 1)
+            newton[nm] = {"mean_per_particle": st_.get("stat_newton", 0) / n,
+                          "mean_wavefront_max": st_.get("stat_newton_wmax", 0) / w}
+        log("Newton iterations:", newton)
         learned = {"particles_per_s": n / dt_l, "ms_per_step": dt_l * 1e3,
+                   "newton_iterations": newton,
                    "speedup_vs_plain": dt_plain / dt_l, "radius_factor": beta,
                    "radius_misses": int(((outl["status"] & 128) != 0).sum()),
                    "newton_seed_rejected": int(((outl["status"] & 32) != 0).sum()),

@@ -134,7 +134,8 @@ int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen) {
                                                      "stat_p2_leaves", "stat_p2_nodes", "stat_waves",
                                                      "stat_lane_need", "stat_maxlane_need",
                                                      "stat_parent_max", "stat_nn_waves", "stat_nn_nodes",
-                                                     "stat_nn_leaves", "stat_nn_exact"};
+                                                     "stat_nn_leaves", "stat_nn_exact", "stat_newton",
+                                                     "stat_newton_wmax"};
         unsigned long long hs[16];
         rc = check_hip(ctx, hipMemcpy(hs, ctx->stats, sizeof hs, hipMemcpyDeviceToHost), "stats read");
         if (rc) return rc;

@@ -138,6 +138,8 @@ enum : int {
     STAT_NN_NODES,        // its wave-uniform node tests (sum over waves)
     STAT_NN_LEAVES,       // its wave-uniform leaf scans (sum over waves)
     STAT_NN_EXACT,        // points re-ranked in float64
+    STAT_NEWTON,          // smoothing-length iterations (sum over query lanes)
+    STAT_NEWTON_WMAX,     // per wavefront, the largest iteration count of a lane (sum)
     STAT_COUNT
 };
 static_assert(STAT_COUNT <= 16, "the work-counter buffer holds 16 entries");

@@ -260,7 +260,7 @@ MHIP_DEV void pad_list(const KnnArgs &a, int cnt, uint32_t qp, int64_t slot) {
 // kernel's register count at that of the traversal.
 // ---------------------------------------------------------------------------
 template <int KCAP>
-MHIP_DEV float hsml_solve(float (&A)[KCAP], int k, int dim, float seed, int32_t &st) {
+MHIP_DEV float hsml_solve(float (&A)[KCAP], int k, int dim, float seed, int32_t &st, int &iters) {
     const int e = KCAP - 1 - k;  // first real slot
     const float norm = (float)hsml_norm(dim);
     float(&r)[KCAP] = A;
@@ -272,6 +272,7 @@ MHIP_DEV float hsml_solve(float (&A)[KCAP], int k, int dim, float seed, int32_t 
         if (j == KCAP - 2) rmax = r[j];
     }
     float hi = (float)hsml_bound_coeff(dim) * rmax;
+    iters = 0;
     if (!(hi > 0.f)) return 0.f;  // all k at distance 0: the reference returns 0
     float h = rmax;
     if (seed > 0.f) {
@@ -325,6 +326,7 @@ MHIP_DEV float hsml_solve(float (&A)[KCAP], int k, int dim, float seed, int32_t 
         h = hn;
     }
     if (it >= 40) st |= ST_HSML_NOCONV;
+    iters = min(it + 1, 40);
     return h;
 }
 
@@ -618,7 +620,18 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
 #pragma unroll
     for (int j = 0; j < KCAP - 1; j++) nlt += (A[j] >= 0.f && A[j] < tau);
     const float seed = a.seed_h ? a.seed_h[row] : 0.f;
-    const float h = hsml_solve<KCAP>(A, a.k, a.dim, seed, st);
+    int n_newton = 0;
+    const float h = hsml_solve<KCAP>(A, a.k, a.dim, seed, st, n_newton);
+    if (a.stats) {
+        // a redo launch counts only the lanes it recomputes for a miss
+        const bool cnt = live && (!a.redo || missed);
+        const uint32_t tot = wave_sum(cnt ? (uint32_t)n_newton : 0u);
+        const uint32_t mx = (uint32_t)wave_max(cnt ? n_newton : 0);
+        if (lane == 0) {
+            atomicAdd(a.stats + STAT_NEWTON, (unsigned long long)tot);
+            atomicAdd(a.stats + STAT_NEWTON_WMAX, (unsigned long long)mx);
+        }
+    }
     if (live) {
         a.tau[slot] = tau;
         a.nlt[slot] = nlt;
