@@ -201,6 +201,20 @@ int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int
 int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
 
 /*
+ * Meshoid.SurfaceDensity (src/meshoid/Meshoid.py:595-643) ->
+ * GridSurfaceDensity (src/meshoid/grid_deposition.py:123-295): the surface
+ * density of f on a res x res grid of sightlines along z, grid_out float64
+ * [gx][gy] with points at center - size/2 + g * size/(res-1).  f (n)
+ * float32/float64, pos (n, dim) with dim 2 or 3 (x, y used), h (n) the
+ * smoothing lengths before np.clip(smooth_fac * h, 2 * size / res, 1e100);
+ * conservative = 1: each particle's weights normalised to f, grid / dx^2
+ * (GridSurfaceDensity_conservative_core).  Accumulated in float64.
+ */
+int mhip_surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
+                         const void *h, int h_f64, double smooth_fac, double cx, double cy, double size, int res,
+                         int conservative, double *grid_out);
+
+/*
  * TreeUpdate with a particle_mask (src/meshoid/Meshoid.py:147-170) served
  * from an unmasked list `full` of the same index and des_ngb: a particle's
  * neighbours, distances and smoothing length do not depend on the mask, so

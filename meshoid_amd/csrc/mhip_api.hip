@@ -348,6 +348,16 @@ int mhip_nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx
     return nlist_subset(ctx, full, qidx, nq, m_rows, h_out, rho_out, vol_out, status_out, nl);
 }
 
+int mhip_surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
+                         const void *h, int h_f64, double smooth_fac, double cx, double cy, double size, int res,
+                         int conservative, double *grid_out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(grid_out && ((f && pos && h) || n == 0), "null argument");
+    hipSetDevice(ctx->device);
+    return surface_density(ctx, f, f_f64 ? 1 : 0, pos, pos_f64 ? 1 : 0, n, dim, h, h_f64 ? 1 : 0, smooth_fac, cx, cy,
+                           size, res, conservative, grid_out);
+}
+
 int mhip_nearest(mhip_ctx *ctx, const mhip_index *ix, const double *points, int64_t nt, int dim,
                  int64_t *nbr_out) {
     if (!ctx) return MHIP_ERR_VALUE;

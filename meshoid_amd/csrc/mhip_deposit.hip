@@ -1,0 +1,263 @@
+// Surface density of a particle quantity on a grid of sightlines.
+//
+// Replaces
+//   Meshoid.SurfaceDensity                   src/meshoid/Meshoid.py:595-643
+//   GridSurfaceDensity (parallel split/sum)  src/meshoid/grid_deposition.py:123-184
+//   GridSurfaceDensity_core                  src/meshoid/grid_deposition.py:187-229
+//   GridSurfaceDensity_conservative_core     src/meshoid/grid_deposition.py:232-295
+//   kernel2d                                 src/meshoid/kernel_density.py:60-68
+//
+// Each particle adds kernel2d(r/h) f / h^2 to every grid point (spacing
+// size/(res-1), point 0 at center - size/2) within h of its projected
+// position; the conservative variant divides each particle's weights by
+// their sum and scales the grid by 1/dx^2 at the end.  Accumulation is
+// float64.  A block keeps a private copy of the grid in LDS (float64 LDS
+// atomics) when it fits, and adds it to the HBM grid once at the end, so the
+// many particles of a clump that land on the same few cells contend in LDS,
+// not in L2.  Particles whose footprint exceeds BIG_CELLS go to a second
+// kernel that spreads one particle over a whole block.
+#include "mhip_internal.hpp"
+
+namespace mhip {
+
+constexpr int DEP_THREADS = 256;
+constexpr int64_t BIG_CELLS = 1024;           // footprint handled by one thread at most
+constexpr int LDS_GRID_MAX = 128 * 128;       // cells of an LDS-resident grid (128 KB)
+constexpr double KERNEL2D_NORM = 1.8189136353359467;  // 40 / (7 pi)
+
+struct DepArgs {
+    const void *f, *pos, *h;
+    int f_f64, pos_f64, h_f64, dim;
+    int64_t n;
+    double smooth, hmin;  // h used = min(max(smooth h, hmin), 1e100)
+    double ox, oy;        // center - size/2: the position of grid point 0
+    double dx;            // size / (res - 1)
+    int res, conservative;
+    double *grid;           // (res, res), [gx][gy]
+    uint32_t *big, *nbig;   // particles left to the block-per-particle kernel
+};
+
+MHIP_DEV double kernel2d(double q) {
+    double k;
+    if (q <= 0.5) {
+        k = 1.0 - 6.0 * q * q * (1.0 - q);
+    } else {
+        const double a = 1.0 - q;
+        k = 2.0 * a * a * a;
+    }
+    return k * KERNEL2D_NORM;
+}
+
+template <typename T>
+MHIP_DEV double ld(const void *p, int64_t i) {
+    return (double)((const T *)p)[i];
+}
+
+struct Foot {
+    double xs0, xs1, hs, hinv, w;  // projected position, h, 1/h, per-weight factor
+    int gx0, gx1, gy0, gy1;
+};
+
+// grid index range of int((x -/+ h) / dx (+ 1)) as the reference forms it
+// (truncation toward zero, then clipped to the grid)
+MHIP_DEV int lo_index(double a, int res) {
+    if (!(a > 0.0)) return 0;
+    return a >= (double)res ? res : (int)a;
+}
+MHIP_DEV int hi_index(double b, int res) {
+    if (!(b > -1.0)) return -1;
+    return b >= (double)(res - 1) ? res - 1 : (int)b;
+}
+
+// false: nothing to deposit (non-finite input or footprint off the grid)
+MHIP_DEV bool footprint(const DepArgs &a, int64_t i, Foot &fp) {
+    const double x0 = a.pos_f64 ? ld<double>(a.pos, i * a.dim) : ld<float>(a.pos, i * a.dim);
+    const double x1 = a.pos_f64 ? ld<double>(a.pos, i * a.dim + 1) : ld<float>(a.pos, i * a.dim + 1);
+    const double hv = a.h_f64 ? ld<double>(a.h, i) : ld<float>(a.h, i);
+    const double fv = a.f_f64 ? ld<double>(a.f, i) : ld<float>(a.f, i);
+    fp.xs0 = (x0 - a.ox);
+    fp.xs1 = (x1 - a.oy);
+    double hs = fmin(fmax(a.smooth * hv, a.hmin), 1e100);
+    if (a.conservative) hs = fmax(hs, a.dx);
+    if (!(hs > 0.0) || !isfinite(fp.xs0) || !isfinite(fp.xs1) || !isfinite(fv)) return false;
+    fp.hs = hs;
+    fp.hinv = 1.0 / hs;
+    fp.w = a.conservative ? fv : fv * fp.hinv * fp.hinv;
+    fp.gx0 = lo_index((fp.xs0 - hs) / a.dx + 1.0, a.res);
+    fp.gx1 = hi_index((fp.xs0 + hs) / a.dx, a.res);
+    fp.gy0 = lo_index((fp.xs1 - hs) / a.dx + 1.0, a.res);
+    fp.gy1 = hi_index((fp.xs1 + hs) / a.dx, a.res);
+    return fp.gx0 <= fp.gx1 && fp.gy0 <= fp.gy1;
+}
+
+// kernel weight at grid point (gx, gy), 0 when it lies beyond h
+MHIP_DEV double cell_weight(const DepArgs &a, const Foot &fp, int gx, int gy) {
+    double ddx = fp.xs0 - gx * a.dx;
+    ddx *= ddx;
+    double ddy = fp.xs1 - gy * a.dx;
+    ddy *= ddy;
+    const double r = sqrt(ddx + ddy);
+    if (r > fp.hs) return 0.0;
+    return kernel2d(r * fp.hinv);
+}
+
+template <bool LDSG>
+MHIP_DEV void add_cell(const DepArgs &a, double *sg, int gx, int gy, double v) {
+    const int c = gx * a.res + gy;
+    if (LDSG)
+        atomicAdd(&sg[c], v);
+    else
+        atomicAdd(&a.grid[c], v);
+}
+
+template <bool LDSG>
+__global__ __launch_bounds__(DEP_THREADS) void k_deposit(DepArgs a) {
+    extern __shared__ double sg[];
+    const int ncell = a.res * a.res;
+    if (LDSG) {
+        for (int c = threadIdx.x; c < ncell; c += DEP_THREADS) sg[c] = 0.0;
+        __syncthreads();
+    }
+    for (int64_t i = blockIdx.x * (int64_t)DEP_THREADS + threadIdx.x; i < a.n;
+         i += (int64_t)gridDim.x * DEP_THREADS) {
+        Foot fp;
+        if (!footprint(a, i, fp)) continue;
+        const int64_t cells = (int64_t)(fp.gx1 - fp.gx0 + 1) * (fp.gy1 - fp.gy0 + 1);
+        if (cells > BIG_CELLS) {
+            // at most one entry per particle: the list holds n entries
+            a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;
+            continue;
+        }
+        // conservative: each weight over the particle's total weight
+        double tot = 1.0;
+        if (a.conservative) {
+            tot = 0.0;
+            for (int gx = fp.gx0; gx <= fp.gx1; gx++)
+                for (int gy = fp.gy0; gy <= fp.gy1; gy++) tot += cell_weight(a, fp, gx, gy);
+            if (tot == 0.0) continue;
+        }
+        for (int gx = fp.gx0; gx <= fp.gx1; gx++)
+            for (int gy = fp.gy0; gy <= fp.gy1; gy++) {
+                const double k = cell_weight(a, fp, gx, gy);
+                if (k != 0.0) add_cell<LDSG>(a, sg, gx, gy, a.conservative ? k * fp.w / tot : k * fp.w);
+            }
+    }
+    if (LDSG) {
+        __syncthreads();
+        for (int c = threadIdx.x; c < ncell; c += DEP_THREADS) {
+            const double v = sg[c];
+            if (v != 0.0) atomicAdd(&a.grid[c], v);
+        }
+    }
+}
+
+// one block per listed particle (footprints above BIG_CELLS), HBM atomics
+__global__ __launch_bounds__(DEP_THREADS) void k_deposit_big(DepArgs a) {
+    __shared__ double red[DEP_THREADS];
+    const uint32_t nb = *a.nbig;
+    for (uint32_t e = blockIdx.x; e < nb; e += gridDim.x) {
+        Foot fp;
+        footprint(a, a.big[e], fp);  // listed, so it has a footprint
+        const int wy = fp.gy1 - fp.gy0 + 1;
+        const int64_t cells = (int64_t)(fp.gx1 - fp.gx0 + 1) * wy;
+        double tot = 1.0;
+        if (a.conservative) {
+            double part = 0.0;
+            for (int64_t c = threadIdx.x; c < cells; c += DEP_THREADS)
+                part += cell_weight(a, fp, fp.gx0 + (int)(c / wy), fp.gy0 + (int)(c % wy));
+            red[threadIdx.x] = part;
+            __syncthreads();
+            for (int o = DEP_THREADS / 2; o > 0; o >>= 1) {
+                if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+                __syncthreads();
+            }
+            tot = red[0];
+            __syncthreads();
+            if (tot == 0.0) continue;
+        }
+        for (int64_t c = threadIdx.x; c < cells; c += DEP_THREADS) {
+            const int gx = fp.gx0 + (int)(c / wy), gy = fp.gy0 + (int)(c % wy);
+            const double k = cell_weight(a, fp, gx, gy);
+            if (k != 0.0) atomicAdd(&a.grid[gx * a.res + gy], a.conservative ? k * fp.w / tot : k * fp.w);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grid_fill(double *__restrict__ g, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        g[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_grid_scale(double *__restrict__ g, int64_t n, double s) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        g[i] *= s;
+}
+
+__global__ void k_u32_zero(uint32_t *p) {
+    if (threadIdx.x == 0) *p = 0u;
+}
+
+int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
+                    const void *h, int h_f64, double smooth, double cx, double cy, double size, int res,
+                    int conservative, double *grid) {
+    if (n < 0 || dim < 2 || dim > 3) return fail(ctx, -1, "surface density needs (n, 2) or (n, 3) positions");
+    if (res < 2) return fail(ctx, -1, "res must be at least 2");
+    if (!(size > 0.0)) return fail(ctx, -1, "size must be positive");
+    if (n > 0xffffffffll) return fail(ctx, -4, "more than 2^32 particles");
+    DepArgs a;
+    a.f = f;
+    a.pos = pos;
+    a.h = h;
+    a.f_f64 = f_f64;
+    a.pos_f64 = pos_f64;
+    a.h_f64 = h_f64;
+    a.dim = dim;
+    a.n = n;
+    a.smooth = smooth;
+    a.hmin = 2.0 * size / res;  // np.clip(smooth_fac * h, 2 * size / res, 1e100)
+    a.ox = cx - 0.5 * size;
+    a.oy = cy - 0.5 * size;
+    a.dx = size / (res - 1);
+    a.res = res;
+    a.conservative = conservative ? 1 : 0;
+    a.grid = grid;
+    int rc = 0;
+    uint32_t *sc = (uint32_t *)scratch(ctx, (size_t)(n + 1) * 4, &rc);
+    if (!sc) return rc;
+    a.nbig = sc;
+    a.big = sc + 1;
+    hipStream_t s = ctx->stream;
+    const int64_t ncell = (int64_t)res * res;
+    ProfRegion pr(ctx, "surface_density");
+    hipLaunchKernelGGL(k_grid_fill, dim3((unsigned)std::min<int64_t>(1024, (ncell + 255) / 256)), dim3(256), 0, s,
+                       grid, ncell, 0.0);
+    hipLaunchKernelGGL(k_u32_zero, dim3(1), dim3(64), 0, s, a.nbig);
+    if (n > 0) {
+        const size_t lds = (size_t)ncell * sizeof(double);
+        bool use_lds = ncell <= LDS_GRID_MAX;
+        if (use_lds && lds > 65536)
+            use_lds = hipFuncSetAttribute((const void *)k_deposit<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds) == hipSuccess;
+        (void)hipGetLastError();
+        if (use_lds) {
+            // one block per CU holds a 128 KB grid; each flushes it once
+            int ncu = 0;
+            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu < 1)
+                ncu = 256;
+            const unsigned nb = (unsigned)std::min<int64_t>(ncu, (n + 255) / 256);
+            hipLaunchKernelGGL(k_deposit<true>, dim3(nb), dim3(DEP_THREADS), lds, s, a);
+        } else {
+            hipLaunchKernelGGL(k_deposit<false>, dim3((unsigned)std::min<int64_t>(8192, (n + 255) / 256)),
+                               dim3(DEP_THREADS), 0, s, a);
+        }
+        hipLaunchKernelGGL(k_deposit_big, dim3(2048), dim3(DEP_THREADS), 0, s, a);
+    }
+    if (conservative) {
+        const double dx2inv = 1.0 / (a.dx * a.dx);
+        hipLaunchKernelGGL(k_grid_scale, dim3((unsigned)std::min<int64_t>(1024, (ncell + 255) / 256)), dim3(256), 0,
+                           s, grid, ncell, dx2inv);
+    }
+    return check_hip(ctx, hipGetLastError(), "surface density launch");
+}
+
+}  // namespace mhip

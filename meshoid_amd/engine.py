@@ -383,3 +383,22 @@ def reconstruct(points, pos, boxsize, nbr, inv, f, d1=None, d2=None, order=1):
                                    int(f.dtype == torch.float64), K, ptr(d1), ptr(d2), int(order), ptr(out)),
               ctx)
     return out
+
+
+def surface_density(f, pos, h, center, size, res=128, smooth_fac=1.0, conservative=False):
+    """GridSurfaceDensity (src/meshoid/grid_deposition.py:123-295) of f at
+    positions pos (n, 2|3) with smoothing lengths h (before the clip to
+    [2 size/res, 1e100] after smooth_fac): (res, res) float64 device grid."""
+    _need(f, (torch.float32, torch.float64), "f")
+    _need(pos, (torch.float32, torch.float64), "pos")
+    _need(h, (torch.float32, torch.float64), "h")
+    n, dim = pos.shape
+    if f.numel() != n or h.numel() != n:
+        raise IndexError("f, positions and smoothing lengths must have one entry per particle")
+    grid = torch.empty((int(res), int(res)), dtype=torch.float64, device=pos.device)
+    ctx = context(pos.device)
+    raise_for(lib.mhip_surface_density(ctx, ptr(f), int(f.dtype == torch.float64), ptr(pos),
+                                       int(pos.dtype == torch.float64), n, dim, ptr(h), int(h.dtype == torch.float64),
+                                       float(smooth_fac), float(center[0]), float(center[1]), float(size), int(res),
+                                       int(bool(conservative)), ptr(grid)), ctx)
+    return grid

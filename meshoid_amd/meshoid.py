@@ -501,3 +501,29 @@ class Meshoid:
         if return_grid:
             return X, Y, Z, f_grid
         return f_grid
+
+    # ---------------------------------------------------------- projection
+    def SurfaceDensity(self, f=None, size=None, center=None, res=128, smooth_fac=1.0, conservative=False):
+        """Surface density of f (default: the particle masses) on a res x res
+        grid of sightlines along z (Meshoid.py:595-643): GridSurfaceDensity of
+        f at the particle positions with smoothing lengths
+        clip(smooth_fac * h, 2 * size / res, 1e100), float64 (res, res)."""
+        if f is None:
+            f = self.m
+        if center is None:
+            center = self.center
+        if size is None:
+            size = self.L
+        if self.dim < 2:
+            raise IndexError("SurfaceDensity projects (x, y): positions need at least 2 dimensions")
+        if self.kernel_radius is None and "kernel_radius" not in self._dev:
+            self.TreeUpdate()
+        dev = self._pos_d.device
+        h = self._dev.get("kernel_radius") if "kernel_radius" not in self._np else None
+        if h is None:
+            h = torch.from_numpy(np.ascontiguousarray(self.kernel_radius, dtype=np.float64)).to(dev)
+        is_t = isinstance(f, torch.Tensor)
+        fd = as_device(f).reshape(-1).contiguous()
+        grid = engine.surface_density(fd, self._pos_d, h.contiguous(), np.asarray(center, dtype=np.float64),
+                                      float(size), int(res), float(smooth_fac), bool(conservative))
+        return grid if is_t else grid.cpu().numpy()

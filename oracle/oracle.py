@@ -19,6 +19,10 @@ meshoid source tree):
 * :func:`kernel_weights`, :func:`kernel_average`, :func:`kernel_variance`,
   :func:`integrate` -- ``get_kernel_weights``, ``KernelAverage``,
   ``KernelVariance`` and ``Integrate``, ``src/meshoid/Meshoid.py:172-177,333-381``.
+* :func:`grid_surface_density`, :func:`kernel2d` -- ``GridSurfaceDensity`` and its
+  serial cores, ``src/meshoid/grid_deposition.py:123-295``, ``kernel2d``
+  ``src/meshoid/kernel_density.py:60-68``; :meth:`OracleMeshoid.SurfaceDensity`
+  (``Meshoid.py:595-643``).
 * :meth:`OracleMeshoid.Reconstruct`, ``Slice``, ``InterpToGrid`` --
   ``src/meshoid/Meshoid.py:383-555`` (the nearest particle from :func:`knn`
   with k = 1, ties to the lowest index).
@@ -176,6 +180,49 @@ def apply_weights(G, ngb, indices, f, row0=0, nrows=None):
     return out.reshape((nq,) + tail + (nrows,))
 
 
+def kernel2d(q):
+    """The normalised 2D cubic spline, 40/(7 pi) (1 - 6 q^2 (1 - q)) for
+    q <= 1/2, 40/(7 pi) 2 (1 - q)^3 for 1/2 < q <= 1 (kernel_density.py:60-68;
+    callers only pass q <= 1)."""
+    q = np.asarray(q, dtype=np.float64)
+    a = 1.0 - q
+    return np.where(q <= 0.5, 1.0 - 6.0 * q * q * a, 2.0 * a * a * a) * 1.8189136353359467
+
+
+def grid_surface_density(f, x, h, center, size, res=100, conservative=False):
+    """GridSurfaceDensity(f, x, h, center, size, res, conservative=...)
+    (grid_deposition.py:123-295).  The parallel path only splits the
+    particles over threads and sums their grids, so the serial cores are
+    restated: grid points g*dx, dx = size/(res-1), on x[:, :2] - center +
+    size/2; each particle deposits kernel2d(r/h) f/h^2 on the points within
+    h (index range int((x - h)/dx + 1) .. int((x + h)/dx), clipped);
+    conservative: h >= dx, weights divided by their sum, grid / dx^2."""
+    f = np.asarray(f, dtype=np.float64).reshape(-1)
+    x2d = np.asarray(x, dtype=np.float64)[:, :2] - np.asarray(center, dtype=np.float64)[:2] + size / 2
+    h = np.asarray(h, dtype=np.float64).reshape(-1)
+    dx = size / (res - 1)
+    grid = np.zeros((res, res))
+    for i in range(len(x2d)):
+        hs = max(h[i], dx) if conservative else h[i]
+        x0, x1 = x2d[i]
+        gx0, gx1 = max(int((x0 - hs) / dx + 1), 0), min(int((x0 + hs) / dx), res - 1)
+        gy0, gy1 = max(int((x1 - hs) / dx + 1), 0), min(int((x1 + hs) / dx), res - 1)
+        if gx1 < gx0 or gy1 < gy0:
+            continue
+        gx = np.arange(gx0, gx1 + 1)
+        gy = np.arange(gy0, gy1 + 1)
+        r = np.sqrt(((x0 - gx * dx) ** 2)[:, None] + ((x1 - gy * dx) ** 2)[None, :])
+        k = np.where(r <= hs, kernel2d(np.minimum(r / hs, 1.0)), 0.0)
+        if conservative:
+            tot = k.sum()
+            if tot == 0:
+                continue
+            grid[gx0:gx1 + 1, gy0:gy1 + 1] += k * f[i] / tot
+        else:
+            grid[gx0:gx1 + 1, gy0:gy1 + 1] += k * (f[i] / hs / hs)
+    return grid / (dx * dx) if conservative else grid
+
+
 class OracleMeshoid:
     """FP64 restatement of the reference ``Meshoid`` hot-path state machine
     (``src/meshoid/Meshoid.py:15-170,237-331``), including its quirks:
@@ -314,3 +361,10 @@ class OracleMeshoid:
         X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
         grid = np.stack((X.ravel(), Y.ravel(), Z.ravel()), axis=1) + center
         return self.Reconstruct(f, grid, order).reshape((res, res, res) + np.shape(f)[1:])
+
+    def SurfaceDensity(self, f=None, size=None, center=None, res=128, smooth_fac=1.0, conservative=False):
+        """Meshoid.py:595-643 (size and center given)."""
+        if f is None:
+            f = self.m
+        h = np.clip(smooth_fac * self.kernel_radius, 2 * size / res, 1e100)
+        return grid_surface_density(f, self.pos, h, center, size, res, conservative)
