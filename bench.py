@@ -268,6 +268,35 @@ def reconstruct_form(args, torch, dev, log, ix, nl_full, pos, pos_d, f, f_d, k, 
                            "f_max_rel": float(err.max()), "f_median_rel": float(np.median(err))}}
 
 
+def surface_density_form(args, torch, dev, log, pos_d, h_d, n, timed):
+    """Meshoid.SurfaceDensity of the particle masses (1/N each) on the
+    headline particles, map centred on the box: res 128 (the grid held in LDS
+    per block) and 1024 (HBM atomics), both deposition modes
+    (src/meshoid/Meshoid.py:595-643, grid_deposition.py:123-295)."""
+    from meshoid_amd import engine
+
+    m = torch.full((n,), 1.0 / n, dtype=torch.float64, device=dev)
+    out = {}
+    for res in (128, 1024):
+        for cons in (False, True):
+            g = {}
+
+            def step():
+                g["g"] = engine.surface_density(m, pos_d, h_d, (50.0, 50.0), 100.0, res, 1.0, cons)
+
+            dt, kern, _ = timed(step, 3, 1, 1)
+            grid = g["g"]
+            dx = 100.0 / (res - 1)
+            tot = float(grid.sum()) * dx * dx  # the mass on the map (1 when every particle lands on it)
+            name = f"res{res}_{'conservative' if cons else 'plain'}"
+            out[name] = {"ms": kern.get("surface_density", dt * 1e3), "particles_per_s": n / (dt or 1e-30),
+                         "map_total_over_mass": tot}
+            log("surface density", name, out[name])
+    # algorithmic bytes per particle: x, y (float32) 8 + h 4 + f 8
+    out["alg_bytes_per_particle"] = 20
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,6 +314,7 @@ def main():
     ap.add_argument("--batch-snapshots", type=int, default=64)
     ap.add_argument("--no-recon", action="store_true", help="skip the InterpToGrid measurement")
     ap.add_argument("--recon-res", type=int, default=128)
+    ap.add_argument("--no-deposit", action="store_true", help="skip the SurfaceDensity measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a known-size kernel "
                          "that scales the profiler's HBM byte counters)")
@@ -443,6 +473,9 @@ def main():
     recon = None
     if not args.no_recon:
         recon = reconstruct_form(args, torch, dev, log, ix, nl, pos, pos_d, f, f_d, k, timed)
+    deposit = None
+    if not args.no_deposit:
+        deposit = surface_density_form(args, torch, dev, log, pos_d, out["h"], n, timed)
     batched = None
     if not args.no_batched:
         batched = batched_form(args, torch, dev, log)
@@ -479,6 +512,7 @@ def main():
         "precision": precision,
         "batched": batched,
         "reconstruct": recon,
+        "surface_density": deposit,
         "cpu_baseline": cpu,
         "library": meshoid_amd._lib.LIB_PATH,
     }

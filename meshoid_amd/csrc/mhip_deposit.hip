@@ -11,18 +11,20 @@
 // size/(res-1), point 0 at center - size/2) within h of its projected
 // position; the conservative variant divides each particle's weights by
 // their sum and scales the grid by 1/dx^2 at the end.  Accumulation is
-// float64.  A block keeps a private copy of the grid in LDS (float64 LDS
-// atomics) when it fits, and adds it to the HBM grid once at the end, so the
-// many particles of a clump that land on the same few cells contend in LDS,
-// not in L2.  Particles whose footprint exceeds BIG_CELLS go to a second
-// kernel that spreads one particle over a whole block.
+// float64.  Up to res = 128 every block keeps a private copy of the whole
+// map in LDS (float64 LDS atomics, 128 KB) for an interleaved share of the
+// particles and adds it to the HBM map once at the end: the many particles
+// of a clump that land on the same few cells contend in LDS, not in L2.
+// Larger maps take float64 atomics in HBM.  Particles whose footprint exceeds
+// BIG_CELLS go to a second kernel that spreads one particle over a whole
+// block.
 #include "mhip_internal.hpp"
 
 namespace mhip {
 
 constexpr int DEP_THREADS = 256;
 constexpr int64_t BIG_CELLS = 1024;           // footprint handled by one thread at most
-constexpr int LDS_GRID_MAX = 128 * 128;       // cells of an LDS-resident grid (128 KB)
+constexpr int DEP_TILE = 128;                 // map tile side held in LDS (128 KB of float64)
 constexpr double KERNEL2D_NORM = 1.8189136353359467;  // 40 / (7 pi)
 
 struct DepArgs {
@@ -101,34 +103,67 @@ MHIP_DEV double cell_weight(const DepArgs &a, const Foot &fp, int gx, int gy) {
     return kernel2d(r * fp.hinv);
 }
 
-template <bool LDSG>
-MHIP_DEV void add_cell(const DepArgs &a, double *sg, int gx, int gy, double v) {
-    const int c = gx * a.res + gy;
-    if (LDSG)
-        atomicAdd(&sg[c], v);
-    else
-        atomicAdd(&a.grid[c], v);
+// block b: map tile (b % ntile) (tw x tw cells held in LDS) and particle
+// chunk (b / ntile), the chunk being every nchunk-th run of 256 particles
+// (interleaved, so that a clump's particles, stored together, spread over
+// all chunks instead of piling onto one block's LDS atomics); it deposits the
+// cells of its tile only, and adds the tile to the HBM map once at the end
+__global__ __launch_bounds__(DEP_THREADS) void k_deposit_tiles(DepArgs a, int tw, int ntx, int nchunk) {
+    extern __shared__ double sg[];
+    const int ntile = ntx * ntx;
+    const int tile = blockIdx.x % ntile, chunk = blockIdx.x / ntile;
+    const int tx0 = (tile / ntx) * tw, ty0 = (tile % ntx) * tw;
+    const int tx1 = min(tx0 + tw, a.res) - 1, ty1 = min(ty0 + tw, a.res) - 1;
+    for (int c = threadIdx.x; c < tw * tw; c += DEP_THREADS) sg[c] = 0.0;
+    __syncthreads();
+    for (int64_t i = (int64_t)chunk * DEP_THREADS + threadIdx.x; i < a.n; i += (int64_t)nchunk * DEP_THREADS) {
+        Foot fp;
+        if (!footprint(a, i, fp)) continue;
+        const int64_t cells = (int64_t)(fp.gx1 - fp.gx0 + 1) * (fp.gy1 - fp.gy0 + 1);
+        if (cells > BIG_CELLS) {
+            // listed once, by the tile that holds its first cell; the list
+            // holds n entries
+            if (fp.gx0 >= tx0 && fp.gx0 <= tx1 && fp.gy0 >= ty0 && fp.gy0 <= ty1)
+                a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;
+            continue;
+        }
+        const int gx0 = max(fp.gx0, tx0), gx1 = min(fp.gx1, tx1);
+        const int gy0 = max(fp.gy0, ty0), gy1 = min(fp.gy1, ty1);
+        if (gx0 > gx1 || gy0 > gy1) continue;
+        // conservative: each weight over the particle's total weight (the
+        // whole footprint, not only this tile's part)
+        double tot = 1.0;
+        if (a.conservative) {
+            tot = 0.0;
+            for (int gx = fp.gx0; gx <= fp.gx1; gx++)
+                for (int gy = fp.gy0; gy <= fp.gy1; gy++) tot += cell_weight(a, fp, gx, gy);
+            if (tot == 0.0) continue;
+        }
+        for (int gx = gx0; gx <= gx1; gx++)
+            for (int gy = gy0; gy <= gy1; gy++) {
+                const double k = cell_weight(a, fp, gx, gy);
+                if (k != 0.0) atomicAdd(&sg[(gx - tx0) * tw + (gy - ty0)], a.conservative ? k * fp.w / tot : k * fp.w);
+            }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < tw * tw; c += DEP_THREADS) {
+        const double v = sg[c];
+        if (v != 0.0) atomicAdd(&a.grid[(int64_t)(tx0 + c / tw) * a.res + (ty0 + c % tw)], v);
+    }
 }
 
-template <bool LDSG>
-__global__ __launch_bounds__(DEP_THREADS) void k_deposit(DepArgs a) {
-    extern __shared__ double sg[];
-    const int ncell = a.res * a.res;
-    if (LDSG) {
-        for (int c = threadIdx.x; c < ncell; c += DEP_THREADS) sg[c] = 0.0;
-        __syncthreads();
-    }
+// maps larger than one tile: every thread deposits its particles straight
+// into the HBM map (float64 atomics), particles interleaved over the grid
+__global__ __launch_bounds__(DEP_THREADS) void k_deposit_hbm(DepArgs a) {
     for (int64_t i = blockIdx.x * (int64_t)DEP_THREADS + threadIdx.x; i < a.n;
          i += (int64_t)gridDim.x * DEP_THREADS) {
         Foot fp;
         if (!footprint(a, i, fp)) continue;
         const int64_t cells = (int64_t)(fp.gx1 - fp.gx0 + 1) * (fp.gy1 - fp.gy0 + 1);
         if (cells > BIG_CELLS) {
-            // at most one entry per particle: the list holds n entries
-            a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;
+            a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;  // at most once per particle
             continue;
         }
-        // conservative: each weight over the particle's total weight
         double tot = 1.0;
         if (a.conservative) {
             tot = 0.0;
@@ -139,15 +174,8 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(DepArgs a) {
         for (int gx = fp.gx0; gx <= fp.gx1; gx++)
             for (int gy = fp.gy0; gy <= fp.gy1; gy++) {
                 const double k = cell_weight(a, fp, gx, gy);
-                if (k != 0.0) add_cell<LDSG>(a, sg, gx, gy, a.conservative ? k * fp.w / tot : k * fp.w);
+                if (k != 0.0) atomicAdd(&a.grid[(int64_t)gx * a.res + gy], a.conservative ? k * fp.w / tot : k * fp.w);
             }
-    }
-    if (LDSG) {
-        __syncthreads();
-        for (int c = threadIdx.x; c < ncell; c += DEP_THREADS) {
-            const double v = sg[c];
-            if (v != 0.0) atomicAdd(&a.grid[c], v);
-        }
     }
 }
 
@@ -233,21 +261,24 @@ int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, in
                        grid, ncell, 0.0);
     hipLaunchKernelGGL(k_u32_zero, dim3(1), dim3(64), 0, s, a.nbig);
     if (n > 0) {
-        const size_t lds = (size_t)ncell * sizeof(double);
-        bool use_lds = ncell <= LDS_GRID_MAX;
-        if (use_lds && lds > 65536)
-            use_lds = hipFuncSetAttribute((const void *)k_deposit<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)lds) == hipSuccess;
-        (void)hipGetLastError();
-        if (use_lds) {
-            // one block per CU holds a 128 KB grid; each flushes it once
-            int ncu = 0;
-            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu < 1)
-                ncu = 256;
-            const unsigned nb = (unsigned)std::min<int64_t>(ncu, (n + 255) / 256);
-            hipLaunchKernelGGL(k_deposit<true>, dim3(nb), dim3(DEP_THREADS), lds, s, a);
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu < 1)
+            ncu = 256;
+        if (res <= DEP_TILE) {
+            // the whole map in LDS: one block per CU, particles interleaved
+            // over the blocks, each block adds its map to HBM once
+            const size_t lds = (size_t)res * res * sizeof(double);
+            if (lds > 65536 && hipFuncSetAttribute((const void *)k_deposit_tiles,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+                return check_hip(ctx, hipGetLastError(), "deposit LDS size");
+            const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(ncu, (n + 255) / 256));
+            hipLaunchKernelGGL(k_deposit_tiles, dim3((unsigned)nchunk), dim3(DEP_THREADS), lds, s, a, res, 1,
+                               (int)nchunk);
         } else {
-            hipLaunchKernelGGL(k_deposit<false>, dim3((unsigned)std::min<int64_t>(8192, (n + 255) / 256)),
+            // per-tile LDS copies would re-read every particle per tile and
+            // pay each wavefront's largest in-tile footprint once per tile
+            // (measured 3x slower at 1024^2): straight HBM atomics instead
+            hipLaunchKernelGGL(k_deposit_hbm, dim3((unsigned)std::min<int64_t>(8192, (n + 255) / 256)),
                                dim3(DEP_THREADS), 0, s, a);
         }
         hipLaunchKernelGGL(k_deposit_big, dim3(2048), dim3(DEP_THREADS), 0, s, a);
