@@ -201,18 +201,33 @@ int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int
 int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
 
 /*
- * Meshoid.SurfaceDensity (src/meshoid/Meshoid.py:595-643) ->
- * GridSurfaceDensity (src/meshoid/grid_deposition.py:123-295): the surface
+ * GridSurfaceDensity (src/meshoid/grid_deposition.py:123-295), behind
+ * Meshoid.SurfaceDensity (src/meshoid/Meshoid.py:595-643): the surface
  * density of f on a res x res grid of sightlines along z, grid_out float64
  * [gx][gy] with points at center - size/2 + g * size/(res-1).  f (n)
- * float32/float64, pos (n, dim) with dim 2 or 3 (x, y used), h (n) the
- * smoothing lengths before np.clip(smooth_fac * h, 2 * size / res, 1e100);
+ * float32/float64, pos (n, dim) with dim 2 or 3 (x, y used), h (n); the
+ * smoothing length used is min(max(smooth_fac * h, hmin), 1e100)
+ * (SurfaceDensity: hmin = 2 size / res; GridSurfaceDensity itself: 1, 0);
  * conservative = 1: each particle's weights normalised to f, grid / dx^2
  * (GridSurfaceDensity_conservative_core).  Accumulated in float64.
  */
 int mhip_surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
-                         const void *h, int h_f64, double smooth_fac, double cx, double cy, double size, int res,
-                         int conservative, double *grid_out);
+                         const void *h, int h_f64, double smooth_fac, double hmin, double cx, double cy, double size,
+                         int res, int conservative, double *grid_out);
+
+/*
+ * GridDensity (src/meshoid/grid_deposition.py:534-603), behind
+ * Meshoid.DepositToGrid (src/meshoid/Meshoid.py:557-593): the density of the
+ * conserved quantity f on a res^3 grid, grid_out float64 [gx][gy][gz] with
+ * points at center - size/2 + dx/2 + g dx, dx = size/(res-1).  Each particle
+ * spreads f over its footprint with weights kernel3d(r/h) / (their sum); the
+ * map is divided by dx^3.  pos (n, 3), h (n) clipped below at hmin
+ * (DepositToGrid: size/(res-1)); periodic = 1 wraps indices (g mod res) as
+ * the reference does for box_size >= 0, else they are clipped.
+ */
+int mhip_grid_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, const void *h,
+                      int h_f64, double hmin, double cx, double cy, double cz, double size, int res, int periodic,
+                      double *grid_out);
 
 /*
  * TreeUpdate with a particle_mask (src/meshoid/Meshoid.py:147-170) served

@@ -6,9 +6,10 @@ importing this package fails if that library is missing.
 """
 from . import engine
 from ._lib import lib as _libhandle
-from .functions import HsmlIter, Kernel, gradient_weights, nearest_image, nearest_image_v
+from .functions import (GridDensity, GridSurfaceDensity, HsmlIter, Kernel, gradient_weights, nearest_image,
+                        nearest_image_v)
 from .meshoid import Meshoid
 
 __version__ = "1.0.0"
 __all__ = ["Meshoid", "HsmlIter", "Kernel", "gradient_weights", "nearest_image",
-           "nearest_image_v", "engine"]
+           "nearest_image_v", "GridSurfaceDensity", "GridDensity", "engine"]

@@ -349,13 +349,23 @@ int mhip_nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx
 }
 
 int mhip_surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
-                         const void *h, int h_f64, double smooth_fac, double cx, double cy, double size, int res,
-                         int conservative, double *grid_out) {
+                         const void *h, int h_f64, double smooth_fac, double hmin, double cx, double cy, double size,
+                         int res, int conservative, double *grid_out) {
     if (!ctx) return MHIP_ERR_VALUE;
     MHIP_NEED(grid_out && ((f && pos && h) || n == 0), "null argument");
     hipSetDevice(ctx->device);
-    return surface_density(ctx, f, f_f64 ? 1 : 0, pos, pos_f64 ? 1 : 0, n, dim, h, h_f64 ? 1 : 0, smooth_fac, cx, cy,
-                           size, res, conservative, grid_out);
+    return surface_density(ctx, f, f_f64 ? 1 : 0, pos, pos_f64 ? 1 : 0, n, dim, h, h_f64 ? 1 : 0, smooth_fac, hmin,
+                           cx, cy, size, res, conservative, grid_out);
+}
+
+int mhip_grid_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, const void *h,
+                      int h_f64, double hmin, double cx, double cy, double cz, double size, int res, int periodic,
+                      double *grid_out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(grid_out && ((f && pos && h) || n == 0), "null argument");
+    hipSetDevice(ctx->device);
+    return grid_density(ctx, f, f_f64 ? 1 : 0, pos, pos_f64 ? 1 : 0, n, h, h_f64 ? 1 : 0, hmin, cx, cy, cz, size, res,
+                        periodic, grid_out);
 }
 
 int mhip_nearest(mhip_ctx *ctx, const mhip_index *ix, const double *points, int64_t nt, int dim,

@@ -6,6 +6,9 @@
 //   GridSurfaceDensity_core                  src/meshoid/grid_deposition.py:187-229
 //   GridSurfaceDensity_conservative_core     src/meshoid/grid_deposition.py:232-295
 //   kernel2d                                 src/meshoid/kernel_density.py:60-68
+//   Meshoid.DepositToGrid                    src/meshoid/Meshoid.py:557-593
+//   GridDensity                              src/meshoid/grid_deposition.py:534-603
+//   kernel3d                                 src/meshoid/kernel_density.py:71-79
 //
 // Each particle adds kernel2d(r/h) f / h^2 to every grid point (spacing
 // size/(res-1), point 0 at center - size/2) within h of its projected
@@ -225,8 +228,196 @@ __global__ void k_u32_zero(uint32_t *p) {
     if (threadIdx.x == 0) *p = 0u;
 }
 
+// ---------------------------------------------------------------------------
+// GridDensity: 3-D conservative deposition.  Grid point g at
+// center - size/2 + dx/2 + g dx (dx = size/(res-1)); each particle spreads f
+// over the points of its cube footprint with weights kernel3d(r/h) / (their
+// sum), and the map is divided by dx^3.  Periodic boxes wrap the indices
+// (g mod res) instead of clipping them.  kernel3d is the compact-support
+// spline: 0 beyond r = h (the reference leaves it unassigned there).
+// ---------------------------------------------------------------------------
+constexpr double KERNEL3D_NORM = 2.546479089470325;  // 8 / pi
+constexpr int64_t BIG_CELLS_3D = 512;
+
+struct Dep3Args {
+    const void *f, *pos, *h;
+    int f_f64, pos_f64, h_f64;
+    int64_t n;
+    double hmin;        // h used = min(max(h, hmin), 1e100)
+    double ox, oy, oz;  // position of grid point 0
+    double dx;
+    int res, periodic;
+    double *grid;  // (res, res, res)
+    uint32_t *big, *nbig;
+};
+
+struct Foot3 {
+    double x[3], hs, hinv, f;
+    int g0[3], g1[3];
+};
+
+MHIP_DEV double kernel3d(double q) {
+    if (q > 1.0) return 0.0;
+    double k;
+    if (q <= 0.5) {
+        k = 1.0 - 6.0 * q * q * (1.0 - q);
+    } else {
+        const double a = 1.0 - q;
+        k = 2.0 * a * a * a;
+    }
+    return k * KERNEL3D_NORM;
+}
+
+MHIP_DEV bool footprint3(const Dep3Args &a, int64_t i, Foot3 &fp) {
+    const double o[3] = {a.ox, a.oy, a.oz};
+    for (int c = 0; c < 3; c++)
+        fp.x[c] = (a.pos_f64 ? ld<double>(a.pos, i * 3 + c) : ld<float>(a.pos, i * 3 + c)) - o[c];
+    const double hv = a.h_f64 ? ld<double>(a.h, i) : ld<float>(a.h, i);
+    fp.f = a.f_f64 ? ld<double>(a.f, i) : ld<float>(a.f, i);
+    const double hs = fmin(fmax(hv, a.hmin), 1e100);
+    if (!(hs > 0.0) || !isfinite(fp.x[0]) || !isfinite(fp.x[1]) || !isfinite(fp.x[2]) || !isfinite(fp.f))
+        return false;
+    fp.hs = hs;
+    fp.hinv = 1.0 / hs;
+    for (int c = 0; c < 3; c++) {
+        const double lo = (fp.x[c] - hs) / a.dx + 1.0, hi = (fp.x[c] + hs) / a.dx;
+        if (a.periodic) {
+            // unclipped int() (truncation toward zero); the footprint spans at
+            // most 2 h / dx + 1 points, h <= the map's extent in practice
+            if (!(fabs(lo) < 1e9 && fabs(hi) < 1e9)) return false;
+            fp.g0[c] = (int)lo;
+            fp.g1[c] = (int)hi;
+        } else {
+            fp.g0[c] = lo_index(lo, a.res);
+            fp.g1[c] = hi_index(hi, a.res);
+        }
+        if (fp.g0[c] > fp.g1[c]) return false;
+    }
+    return true;
+}
+
+MHIP_DEV double weight3(const Dep3Args &a, const Foot3 &fp, int gx, int gy, int gz) {
+    double dx2 = fp.x[0] - gx * a.dx;
+    dx2 *= dx2;
+    double dy2 = fp.x[1] - gy * a.dx;
+    dy2 *= dy2;
+    double dz2 = fp.x[2] - gz * a.dx;
+    dz2 *= dz2;
+    return kernel3d(sqrt(dx2 + dy2 + dz2) * fp.hinv);
+}
+
+MHIP_DEV int64_t cell3(const Dep3Args &a, int gx, int gy, int gz) {
+    if (a.periodic) {
+        gx = ((gx % a.res) + a.res) % a.res;
+        gy = ((gy % a.res) + a.res) % a.res;
+        gz = ((gz % a.res) + a.res) % a.res;
+    }
+    return ((int64_t)gx * a.res + gy) * a.res + gz;
+}
+
+__global__ __launch_bounds__(DEP_THREADS) void k_density3d(Dep3Args a) {
+    for (int64_t i = blockIdx.x * (int64_t)DEP_THREADS + threadIdx.x; i < a.n;
+         i += (int64_t)gridDim.x * DEP_THREADS) {
+        Foot3 fp;
+        if (!footprint3(a, i, fp)) continue;
+        const int64_t cells = (int64_t)(fp.g1[0] - fp.g0[0] + 1) * (fp.g1[1] - fp.g0[1] + 1) * (fp.g1[2] - fp.g0[2] + 1);
+        if (cells > BIG_CELLS_3D) {
+            a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;  // at most once per particle
+            continue;
+        }
+        double tot = 0.0;
+        for (int gx = fp.g0[0]; gx <= fp.g1[0]; gx++)
+            for (int gy = fp.g0[1]; gy <= fp.g1[1]; gy++)
+                for (int gz = fp.g0[2]; gz <= fp.g1[2]; gz++) tot += weight3(a, fp, gx, gy, gz);
+        if (!(tot > 0.0)) continue;
+        for (int gx = fp.g0[0]; gx <= fp.g1[0]; gx++)
+            for (int gy = fp.g0[1]; gy <= fp.g1[1]; gy++)
+                for (int gz = fp.g0[2]; gz <= fp.g1[2]; gz++) {
+                    const double k = weight3(a, fp, gx, gy, gz);
+                    if (k != 0.0) atomicAdd(&a.grid[cell3(a, gx, gy, gz)], fp.f * k / tot);
+                }
+    }
+}
+
+// one block per listed particle (footprints above BIG_CELLS_3D)
+__global__ __launch_bounds__(DEP_THREADS) void k_density3d_big(Dep3Args a) {
+    __shared__ double red[DEP_THREADS];
+    const uint32_t nb = *a.nbig;
+    for (uint32_t e = blockIdx.x; e < nb; e += gridDim.x) {
+        Foot3 fp;
+        footprint3(a, a.big[e], fp);  // listed, so it has a footprint
+        const int wy = fp.g1[1] - fp.g0[1] + 1, wz = fp.g1[2] - fp.g0[2] + 1;
+        const int64_t cells = (int64_t)(fp.g1[0] - fp.g0[0] + 1) * wy * wz;
+        double part = 0.0;
+        for (int64_t c = threadIdx.x; c < cells; c += DEP_THREADS)
+            part += weight3(a, fp, fp.g0[0] + (int)(c / ((int64_t)wy * wz)), fp.g0[1] + (int)((c / wz) % wy),
+                            fp.g0[2] + (int)(c % wz));
+        red[threadIdx.x] = part;
+        __syncthreads();
+        for (int o = DEP_THREADS / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        const double tot = red[0];
+        __syncthreads();
+        if (!(tot > 0.0)) continue;
+        for (int64_t c = threadIdx.x; c < cells; c += DEP_THREADS) {
+            const int gx = fp.g0[0] + (int)(c / ((int64_t)wy * wz)), gy = fp.g0[1] + (int)((c / wz) % wy),
+                      gz = fp.g0[2] + (int)(c % wz);
+            const double k = weight3(a, fp, gx, gy, gz);
+            if (k != 0.0) atomicAdd(&a.grid[cell3(a, gx, gy, gz)], fp.f * k / tot);
+        }
+    }
+}
+
+int grid_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, const void *h,
+                 int h_f64, double hmin, double cx, double cy, double cz, double size, int res, int periodic,
+                 double *grid) {
+    if (n < 0) return fail(ctx, -1, "negative particle count");
+    if (res < 2 || res > 2048) return fail(ctx, -1, "res must be in [2, 2048]");
+    if (!(size > 0.0)) return fail(ctx, -1, "size must be positive");
+    if (n > 0xffffffffll) return fail(ctx, -4, "more than 2^32 particles");
+    Dep3Args a;
+    a.f = f;
+    a.pos = pos;
+    a.h = h;
+    a.f_f64 = f_f64;
+    a.pos_f64 = pos_f64;
+    a.h_f64 = h_f64;
+    a.n = n;
+    a.hmin = hmin;
+    a.dx = size / (res - 1);
+    // x - center + size/2 - dx/2: point g at center - size/2 + dx/2 + g dx
+    a.ox = cx - 0.5 * size + 0.5 * a.dx;
+    a.oy = cy - 0.5 * size + 0.5 * a.dx;
+    a.oz = cz - 0.5 * size + 0.5 * a.dx;
+    a.res = res;
+    a.periodic = periodic ? 1 : 0;
+    a.grid = grid;
+    int rc = 0;
+    uint32_t *sc = (uint32_t *)scratch(ctx, (size_t)(n + 1) * 4, &rc);
+    if (!sc) return rc;
+    a.nbig = sc;
+    a.big = sc + 1;
+    hipStream_t s = ctx->stream;
+    const int64_t ncell = (int64_t)res * res * res;
+    ProfRegion pr(ctx, "grid_density");
+    hipLaunchKernelGGL(k_grid_fill, dim3((unsigned)std::min<int64_t>(4096, (ncell + 255) / 256)), dim3(256), 0, s,
+                       grid, ncell, 0.0);
+    hipLaunchKernelGGL(k_u32_zero, dim3(1), dim3(64), 0, s, a.nbig);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_density3d, dim3((unsigned)std::min<int64_t>(8192, (n + 255) / 256)), dim3(DEP_THREADS),
+                           0, s, a);
+        hipLaunchKernelGGL(k_density3d_big, dim3(2048), dim3(DEP_THREADS), 0, s, a);
+    }
+    const double inv = 1.0 / (a.dx * a.dx * a.dx);
+    hipLaunchKernelGGL(k_grid_scale, dim3((unsigned)std::min<int64_t>(4096, (ncell + 255) / 256)), dim3(256), 0, s,
+                       grid, ncell, inv);
+    return check_hip(ctx, hipGetLastError(), "grid density launch");
+}
+
 int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
-                    const void *h, int h_f64, double smooth, double cx, double cy, double size, int res,
+                    const void *h, int h_f64, double smooth, double hmin, double cx, double cy, double size, int res,
                     int conservative, double *grid) {
     if (n < 0 || dim < 2 || dim > 3) return fail(ctx, -1, "surface density needs (n, 2) or (n, 3) positions");
     if (res < 2) return fail(ctx, -1, "res must be at least 2");
@@ -242,7 +433,7 @@ int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, in
     a.dim = dim;
     a.n = n;
     a.smooth = smooth;
-    a.hmin = 2.0 * size / res;  // np.clip(smooth_fac * h, 2 * size / res, 1e100)
+    a.hmin = hmin;  // SurfaceDensity: np.clip(smooth_fac * h, 2 * size / res, 1e100)
     a.ox = cx - 0.5 * size;
     a.oy = cy - 0.5 * size;
     a.dx = size / (res - 1);

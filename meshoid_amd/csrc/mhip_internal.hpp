@@ -61,8 +61,11 @@ int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int6
 int unique_inverse(mhip_ctx *ctx, const int64_t *nbr, int64_t nt, int64_t n, int64_t *uniq, int64_t *inv,
                    int64_t *nuniq);
 int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
-                    const void *h, int h_f64, double smooth, double cx, double cy, double size, int res,
+                    const void *h, int h_f64, double smooth, double hmin, double cx, double cy, double size, int res,
                     int conservative, double *grid);
+int grid_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, const void *h,
+                 int h_f64, double hmin, double cx, double cy, double cz, double size, int res, int periodic,
+                 double *grid);
 int reconstruct(mhip_ctx *ctx, const double *targets, int64_t nt, int dim, double box, const void *pos,
                 int pos_f64, const int64_t *nbr, const int64_t *inv, const void *f, int f_f64, int K,
                 const float *d1, const float *d2, int order, void *out);

@@ -385,10 +385,11 @@ def reconstruct(points, pos, boxsize, nbr, inv, f, d1=None, d2=None, order=1):
     return out
 
 
-def surface_density(f, pos, h, center, size, res=128, smooth_fac=1.0, conservative=False):
+def surface_density(f, pos, h, center, size, res=128, smooth_fac=1.0, conservative=False, hmin=None):
     """GridSurfaceDensity (src/meshoid/grid_deposition.py:123-295) of f at
-    positions pos (n, 2|3) with smoothing lengths h (before the clip to
-    [2 size/res, 1e100] after smooth_fac): (res, res) float64 device grid."""
+    positions pos (n, 2|3) with smoothing lengths clip(smooth_fac h, hmin,
+    1e100), hmin defaulting to SurfaceDensity's 2 size/res:
+    (res, res) float64 device grid."""
     _need(f, (torch.float32, torch.float64), "f")
     _need(pos, (torch.float32, torch.float64), "pos")
     _need(h, (torch.float32, torch.float64), "h")
@@ -399,6 +400,29 @@ def surface_density(f, pos, h, center, size, res=128, smooth_fac=1.0, conservati
     ctx = context(pos.device)
     raise_for(lib.mhip_surface_density(ctx, ptr(f), int(f.dtype == torch.float64), ptr(pos),
                                        int(pos.dtype == torch.float64), n, dim, ptr(h), int(h.dtype == torch.float64),
-                                       float(smooth_fac), float(center[0]), float(center[1]), float(size), int(res),
+                                       float(smooth_fac), float(2.0 * size / res if hmin is None else hmin),
+                                       float(center[0]), float(center[1]), float(size), int(res),
                                        int(bool(conservative)), ptr(grid)), ctx)
+    return grid
+
+
+def grid_density(f, pos, h, center, size, res=128, periodic=False, hmin=0.0):
+    """GridDensity (src/meshoid/grid_deposition.py:534-603): (res, res, res)
+    float64 device grid of the density of f, smoothing lengths clipped below
+    at hmin; periodic wraps the grid indices."""
+    _need(f, (torch.float32, torch.float64), "f")
+    _need(pos, (torch.float32, torch.float64), "pos")
+    _need(h, (torch.float32, torch.float64), "h")
+    n = pos.shape[0]
+    if pos.dim() != 2 or pos.shape[1] != 3:
+        raise ValueError("GridDensity needs (N, 3) positions")
+    if f.numel() != n or h.numel() != n:
+        raise IndexError("f, positions and smoothing lengths must have one entry per particle")
+    grid = torch.empty((int(res),) * 3, dtype=torch.float64, device=pos.device)
+    c = [float(v) for v in center]
+    ctx = context(pos.device)
+    raise_for(lib.mhip_grid_density(ctx, ptr(f), int(f.dtype == torch.float64), ptr(pos),
+                                    int(pos.dtype == torch.float64), n, ptr(h), int(h.dtype == torch.float64),
+                                    float(hmin), c[0], c[1], c[2], float(size), int(res), int(bool(periodic)),
+                                    ptr(grid)), ctx)
     return grid

@@ -98,3 +98,33 @@ def gradient_weights(pos, ngb, kernel_radius, indices, boxsize=None, weighted=Tr
     if bool(((status & ST_SINGULAR) != 0).any()):
         raise np.linalg.LinAlgError("Matrix is singular to machine precision.")
     return _out(G, pos)
+
+
+def _dev_like(a, dtypes=(torch.float32, torch.float64)):
+    return as_device(a, dtypes).contiguous()
+
+
+def GridSurfaceDensity(f, x, h, center, size, res=100, box_size=-1, parallel=True, conservative=False):
+    """Surface density of the conserved quantity f at positions x (N, 2|3)
+    with smoothing lengths h on a res x res grid of sightlines along z
+    (grid_deposition.py:123-295): float64 (res, res), points at
+    center - size/2 + g size/(res-1).  box_size and parallel are accepted and,
+    as in the reference's cores, unused."""
+    pos = _dev_like(x)
+    if pos.dim() != 2 or pos.shape[1] < 2:
+        raise IndexError("GridSurfaceDensity needs (N, 2) or (N, 3) positions")
+    g = engine.surface_density(_dev_like(f).reshape(-1), pos, _dev_like(h).reshape(-1),
+                               np.asarray(center, dtype=np.float64), float(size), int(res), 1.0,
+                               bool(conservative), hmin=0.0)
+    return _out(g, x)
+
+
+def GridDensity(f, x, h, center, size, res=100, box_size=-1.0):
+    """Density of the conserved quantity f on a res^3 grid
+    (grid_deposition.py:534-603): float64 (res, res, res), points at
+    center - size/2 + dx/2 + g dx, dx = size/(res-1); box_size >= 0 wraps the
+    grid indices periodically, as the reference does."""
+    periodic = box_size is not None and box_size >= 0
+    g = engine.grid_density(_dev_like(f).reshape(-1), _dev_like(x), _dev_like(h).reshape(-1),
+                            np.asarray(center, dtype=np.float64), float(size), int(res), periodic, 0.0)
+    return _out(g, x)

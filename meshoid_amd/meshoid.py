@@ -527,3 +527,28 @@ class Meshoid:
         grid = engine.surface_density(fd, self._pos_d, h.contiguous(), np.asarray(center, dtype=np.float64),
                                       float(size), int(res), float(smooth_fac), bool(conservative))
         return grid if is_t else grid.cpu().numpy()
+
+    def DepositToGrid(self, f, weights=None, size=None, center=None, res=128):
+        """Density of the conserved quantity f on a res^3 grid
+        (Meshoid.py:557-593): GridDensity with h = clip(h, size/(res-1), 1e100),
+        indices wrapped periodically when the meshoid has a box.  ``weights``
+        is accepted and unused, as in the reference.  Without a box the
+        reference's call fails in its compiled kernel (box_size None); here the
+        grid indices are clipped, as GridDensity does for box_size < 0."""
+        if center is None:
+            center = self.center
+        if size is None:
+            size = self.L
+        if self.dim != 3:
+            raise ValueError("DepositToGrid needs 3D positions")
+        if self.kernel_radius is None and "kernel_radius" not in self._dev:
+            self.TreeUpdate()
+        dev = self._pos_d.device
+        h = self._dev.get("kernel_radius") if "kernel_radius" not in self._np else None
+        if h is None:
+            h = torch.from_numpy(np.ascontiguousarray(self.kernel_radius, dtype=np.float64)).to(dev)
+        is_t = isinstance(f, torch.Tensor)
+        fd = as_device(f).reshape(-1).contiguous()
+        g = engine.grid_density(fd, self._pos_d, h.contiguous(), np.asarray(center, dtype=np.float64), float(size),
+                                int(res), self.boxsize is not None, float(size) / (res - 1))
+        return g if is_t else g.cpu().numpy()

@@ -23,6 +23,10 @@ meshoid source tree):
   serial cores, ``src/meshoid/grid_deposition.py:123-295``, ``kernel2d``
   ``src/meshoid/kernel_density.py:60-68``; :meth:`OracleMeshoid.SurfaceDensity`
   (``Meshoid.py:595-643``).
+* :func:`grid_density`, :func:`kernel3d` -- ``GridDensity``
+  ``src/meshoid/grid_deposition.py:534-603``, ``kernel3d``
+  ``src/meshoid/kernel_density.py:71-79``; :meth:`OracleMeshoid.DepositToGrid`
+  (``Meshoid.py:557-593``).
 * :meth:`OracleMeshoid.Reconstruct`, ``Slice``, ``InterpToGrid`` --
   ``src/meshoid/Meshoid.py:383-555`` (the nearest particle from :func:`knn`
   with k = 1, ties to the lowest index).
@@ -223,6 +227,48 @@ def grid_surface_density(f, x, h, center, size, res=100, conservative=False):
     return grid / (dx * dx) if conservative else grid
 
 
+def kernel3d(q):
+    """The normalised 3D cubic spline, 8/pi (1 - 6 q^2 (1 - q)) for q <= 1/2,
+    8/pi 2 (1 - q)^3 for 1/2 < q <= 1 (kernel_density.py:71-79), and 0 beyond
+    q = 1, where the reference leaves the value unassigned."""
+    q = np.asarray(q, dtype=np.float64)
+    a = 1.0 - q
+    k = np.where(q <= 0.5, 1.0 - 6.0 * q * q * a, 2.0 * a * a * a) * 2.546479089470325
+    return np.where(q <= 1.0, k, 0.0)
+
+
+def grid_density(f, x, h, center, size, res=100, box_size=-1.0):
+    """GridDensity (grid_deposition.py:534-603): grid points at
+    center - size/2 + dx/2 + g dx, dx = size/(res-1); each particle spreads f
+    over its cube footprint (index range int((x - h)/dx + 1) ..
+    int((x + h)/dx), clipped unless box_size >= 0, where indices wrap mod
+    res) with weights kernel3d(r/h) / (their sum); the grid is divided by dx^3."""
+    f = np.asarray(f, dtype=np.float64).reshape(-1)
+    h = np.asarray(h, dtype=np.float64).reshape(-1)
+    dx = size / (res - 1)
+    x3d = np.asarray(x, dtype=np.float64) - np.asarray(center, dtype=np.float64) + size / 2 - dx / 2
+    periodic = box_size is not None and box_size >= 0
+    grid = np.zeros((res, res, res))
+    for i in range(len(x3d)):
+        hs = h[i]
+        ranges = []
+        for c in range(3):
+            lo, hi = int((x3d[i, c] - hs) / dx + 1), int((x3d[i, c] + hs) / dx)
+            if not periodic:
+                lo, hi = max(lo, 0), min(hi, res - 1)
+            ranges.append(np.arange(lo, hi + 1))
+        if any(len(r) == 0 for r in ranges):
+            continue
+        d2 = [(x3d[i, c] - ranges[c] * dx) ** 2 for c in range(3)]
+        r = np.sqrt(d2[0][:, None, None] + d2[1][None, :, None] + d2[2][None, None, :])
+        k = kernel3d(r / hs)
+        tot = k.sum()
+        if tot > 0:
+            gx, gy, gz = np.meshgrid(ranges[0] % res, ranges[1] % res, ranges[2] % res, indexing="ij")
+            np.add.at(grid, (gx, gy, gz), f[i] * k / tot)
+    return grid / (dx * dx * dx)
+
+
 class OracleMeshoid:
     """FP64 restatement of the reference ``Meshoid`` hot-path state machine
     (``src/meshoid/Meshoid.py:15-170,237-331``), including its quirks:
@@ -368,3 +414,8 @@ class OracleMeshoid:
             f = self.m
         h = np.clip(smooth_fac * self.kernel_radius, 2 * size / res, 1e100)
         return grid_surface_density(f, self.pos, h, center, size, res, conservative)
+
+    def DepositToGrid(self, f, size, center, res=128):
+        """Meshoid.py:557-593 (size and center given)."""
+        h = np.clip(self.kernel_radius, size / (res - 1), 1e100)
+        return grid_density(f, self.pos, h, center, size, res, -1.0 if self.boxsize is None else self.boxsize)

@@ -59,3 +59,34 @@ def test_surface_density_clip():
     # every h clipped up to 2 size / res
     want = orc.grid_surface_density(O.m, pos, np.full(500, 2 * size / res), [0.5, 0.5], size, res)
     assert np.array_equal(s, want)
+
+
+def test_kernel3d_normalised():
+    r = (np.arange(200000) + 0.5) / 200000
+    assert abs(np.sum(orc.kernel3d(r) * 4 * np.pi * r * r) / 200000 - 1.0) < 1e-9
+    assert orc.kernel3d(1.5) == 0.0 and np.isclose(orc.kernel3d(0.0), 8 / np.pi)
+
+
+def test_grid_density_conserves_and_points():
+    rng = np.random.default_rng(2)
+    n = 200
+    size, res = 10.0, 24
+    dx = size / (res - 1)
+    x = rng.random((n, 3)) * 6 + 2
+    f = rng.random(n) + 0.1
+    h = rng.random(n) * 1.5 + dx
+    g = orc.grid_density(f, x, h, [5.0, 5.0, 5.0], size, res)
+    assert abs(g.sum() * dx**3 - f.sum()) < 1e-12 * f.sum()
+    # on grid point (10, 11, 12) with h = dx: the six neighbours sit at q = 1
+    # (weight 0), so all of f lands on that point
+    p = np.array([[10.0, 11.0, 12.0]]) * dx + 5.0 - size / 2 + dx / 2
+    g = orc.grid_density([2.0], p, [dx], [5.0, 5.0, 5.0], size, res)
+    assert np.isclose(g[10, 11, 12], 2.0 / dx**3, rtol=1e-12) and np.count_nonzero(g > 1e-12 * g.max()) == 1
+    # periodic: a particle at the corner wraps onto the far faces and keeps its mass
+    g = orc.grid_density([1.0], [[0.0, 0.0, 0.0]], [2.5 * dx], [5.0, 5.0, 5.0], size, res, box_size=size)
+    assert abs(g.sum() * dx**3 - 1.0) < 1e-12 and g[-1, -1, -1] > 0 and g[0, 0, 0] > 0
+    # clipped (open) grid: the weights are normalised over the points on the
+    # grid, so the mass stays whole, all of it in the corner
+    g2 = orc.grid_density([1.0], [[0.0, 0.0, 0.0]], [2.5 * dx], [5.0, 5.0, 5.0], size, res)
+    assert abs(g2.sum() * dx**3 - 1.0) < 1e-12 and g2[-1, -1, -1] == 0.0
+    assert np.count_nonzero(g2[:3, :3, :3]) == np.count_nonzero(g2)

@@ -83,3 +83,56 @@ def test_meshoid_surface_density_2d_and_3d(oracle):
     inside = np.all(np.abs(pos[:, :2] - M.center[:2]) < M.L / 2 - 2 * M.L / 64, axis=1)
     # the conservative map holds at least the mass of the particles well inside
     assert s.sum() * dx * dx >= 2.0 * inside.sum() * (1 - 1e-9)
+
+
+def test_grid_density_matches_oracle(oracle):
+    """GridDensity / DepositToGrid (grid_deposition.py:534-603,
+    Meshoid.py:557-593): open and periodic grids, float32 and float64, large
+    footprints on the block-per-particle kernel."""
+    torch = _torch()
+    import meshoid_amd
+    from meshoid_amd import engine
+
+    rng = np.random.default_rng(5)
+    n = 3000
+    pos = rng.random((n, 3)) * 11 - 0.5
+    f = rng.random(n) + 0.1
+    h = np.exp(rng.normal(-1.0, 0.6, n))
+    h[:10] = 3.5  # footprints over 512 points at res 24 / 40
+    c = np.array([5.0, 5.0, 5.0])
+    for res in (24, 40):
+        for periodic in (False, True):
+            for dt in (np.float64, np.float32):
+                say("grid density res", res, "periodic", periodic, dt.__name__)
+                g = engine.grid_density(torch.from_numpy(f.astype(dt)).cuda(), torch.from_numpy(pos.astype(dt)).cuda(),
+                                        torch.from_numpy(h.astype(dt)).cuda(), c, 10.0, res, periodic, 0.0)
+                go = oracle.grid_density(f.astype(dt), pos.astype(dt), h.astype(dt), c, 10.0, res,
+                                         10.0 if periodic else -1.0)
+                _close(g.cpu().numpy(), go, f"grid density res {res} periodic {periodic} {dt.__name__}")
+    # the module function and the Meshoid method (periodic box, as the reference needs)
+    g = meshoid_amd.GridDensity(f, pos, h, c, 10.0, res=24, box_size=10.0)
+    _close(g, oracle.grid_density(f, pos, h, c, 10.0, 24, 10.0), "GridDensity module function")
+    pb = np.mod(rng.random((20000, 3)) * 10.0, 10.0)
+    M = meshoid_amd.Meshoid(pb, des_ngb=32, boxsize=10.0)
+    O = oracle.OracleMeshoid(pb, des_ngb=32, boxsize=10.0)
+    m = rng.random(20000)
+    d = M.DepositToGrid(m, size=10.0, center=c, res=32)
+    do = O.DepositToGrid(m, 10.0, c, res=32)
+    err = np.abs(d - do).max() / np.abs(do).max()
+    say("DepositToGrid max rel to map max", err)
+    assert d.shape == (32, 32, 32) and err <= 1e-5  # the float32 h
+    dx = 10.0 / 31
+    assert abs(d.sum() * dx**3 / m.sum() - 1.0) < 1e-9  # periodic: every particle's mass on the grid
+
+
+def test_grid_surface_density_module_function(oracle):
+    _torch()
+    import meshoid_amd
+
+    rng = np.random.default_rng(6)
+    pos = rng.random((4000, 3)) * 10
+    f = rng.random(4000)
+    h = rng.random(4000) * 0.3 + 0.01  # no clip: small h deposit on few or no points
+    for cons in (False, True):
+        g = meshoid_amd.GridSurfaceDensity(f, pos, h, [5.0, 5.0], 10.0, res=64, conservative=cons)
+        _close(g, oracle.grid_surface_density(f, pos, h, [5.0, 5.0], 10.0, 64, cons), f"module cons {cons}")
