@@ -294,6 +294,17 @@ def surface_density_form(args, torch, dev, log, pos_d, h_d, n, timed):
             log("surface density", name, out[name])
     # algorithmic bytes per particle: x, y (float32) 8 + h 4 + f 8
     out["alg_bytes_per_particle"] = 20
+    # DepositToGrid / GridDensity: 128^3 over the box, h >= dx
+    g = {}
+
+    def step3():
+        g["g"] = engine.grid_density(m, pos_d, h_d, (50.0, 50.0, 50.0), 100.0, 128, False, 100.0 / 127)
+
+    dt, kern, _ = timed(step3, 3, 1, 1)
+    dx = 100.0 / 127
+    out["grid_density_res128"] = {"ms": kern.get("grid_density", dt * 1e3), "particles_per_s": n / (dt or 1e-30),
+                                  "map_total_over_mass": float(g["g"].sum()) * dx**3}
+    log("grid density res128", out["grid_density_res128"])
     return out
 
 
