@@ -216,6 +216,20 @@ int mhip_surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *po
                          int res, int conservative, double *grid_out);
 
 /*
+ * radtransfer (src/meshoid/radiation/radtransfer.py:11-177): intensity along
+ * z through emitting, absorbing particles on a gridres x gridres grid of
+ * sightlines at center + (-L/2 + g L/(gridres-1)) in x and y.  All arrays
+ * float64: j (n, nf) emissivity per unit mass, m (n), kappa (n, nf) opacity,
+ * x (n, 3), h (n); i0 (gridres, gridres, nf) background or NULL (0).  The
+ * particles act farthest first (largest z first), as
+ * radtransfer_singlethread orders them.  image_out float64 (gridres,
+ * gridres, nf).  nf <= 64.
+ */
+int mhip_radtransfer(mhip_ctx *ctx, const double *j, const double *m, const double *kappa, const double *x,
+                     const double *h, int64_t n, int nf, int gridres, double L, double cx, double cy, const double *i0,
+                     double *image_out);
+
+/*
  * GridDensity (src/meshoid/grid_deposition.py:534-603), behind
  * Meshoid.DepositToGrid (src/meshoid/Meshoid.py:557-593): the density of the
  * conserved quantity f on a res^3 grid, grid_out float64 [gx][gy][gz] with

@@ -9,7 +9,8 @@ from ._lib import lib as _libhandle
 from .functions import (GridDensity, GridSurfaceDensity, HsmlIter, Kernel, gradient_weights, nearest_image,
                         nearest_image_v)
 from .meshoid import Meshoid
+from .radiation import radtransfer
 
 __version__ = "1.0.0"
 __all__ = ["Meshoid", "HsmlIter", "Kernel", "gradient_weights", "nearest_image",
-           "nearest_image_v", "GridSurfaceDensity", "GridDensity", "engine"]
+           "nearest_image_v", "GridSurfaceDensity", "GridDensity", "radtransfer", "engine"]

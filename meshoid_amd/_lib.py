@@ -30,7 +30,7 @@ EXPORTED = [
     "mhip_kernel_weights", "mhip_kernel_average", "mhip_kernel_variance", "mhip_integrate",
     "mhip_batch_create", "mhip_batch_run", "mhip_batch_info", "mhip_batch_destroy",
     "mhip_index_order", "mhip_nearest", "mhip_unique_inverse", "mhip_reconstruct", "mhip_nlist_subset",
-    "mhip_surface_density", "mhip_grid_density",
+    "mhip_surface_density", "mhip_grid_density", "mhip_radtransfer",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -86,6 +86,7 @@ _SIGS = {
     "mhip_nlist_subset": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _p], _i),
     "mhip_surface_density": ([_p, _p, _i, _p, _i, _i64, _i, _p, _i, _d, _d, _d, _d, _d, _i, _i, _p], _i),
     "mhip_grid_density": ([_p, _p, _i, _p, _i, _i64, _p, _i, _d, _d, _d, _d, _d, _i, _i, _p], _i),
+    "mhip_radtransfer": ([_p, _p, _p, _p, _p, _p, _i64, _i, _i, _d, _d, _d, _p, _p], _i),
     "mhip_unique_inverse": ([_p, _p, _i64, _i64, _p, _p, _p], _i),
     "mhip_reconstruct": ([_p, _p, _i64, _i, _d, _p, _i, _p, _p, _p, _i, _i, _p, _p, _i, _p], _i),
 }

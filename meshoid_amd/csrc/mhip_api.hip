@@ -358,6 +358,15 @@ int mhip_surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *po
                            cx, cy, size, res, conservative, grid_out);
 }
 
+int mhip_radtransfer(mhip_ctx *ctx, const double *j, const double *m, const double *kappa, const double *x,
+                     const double *h, int64_t n, int nf, int gridres, double L, double cx, double cy, const double *i0,
+                     double *image_out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(image_out && ((j && m && kappa && x && h) || n == 0), "null argument");
+    hipSetDevice(ctx->device);
+    return radtransfer(ctx, j, m, kappa, x, h, n, nf, gridres, L, cx, cy, i0, image_out);
+}
+
 int mhip_grid_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, const void *h,
                       int h_f64, double hmin, double cx, double cy, double cz, double size, int res, int periodic,
                       double *grid_out) {

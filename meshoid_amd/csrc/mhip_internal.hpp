@@ -63,6 +63,9 @@ int unique_inverse(mhip_ctx *ctx, const int64_t *nbr, int64_t nt, int64_t n, int
 int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
                     const void *h, int h_f64, double smooth, double hmin, double cx, double cy, double size, int res,
                     int conservative, double *grid);
+int radtransfer(mhip_ctx *ctx, const double *j, const double *m, const double *kappa, const double *x,
+                const double *h, int64_t n, int nf, int res, double L, double cx, double cy, const double *i0,
+                double *out);
 int grid_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, const void *h,
                  int h_f64, double hmin, double cx, double cy, double cz, double size, int res, int periodic,
                  double *grid);

@@ -27,6 +27,8 @@ meshoid source tree):
   ``src/meshoid/grid_deposition.py:534-603``, ``kernel3d``
   ``src/meshoid/kernel_density.py:71-79``; :meth:`OracleMeshoid.DepositToGrid`
   (``Meshoid.py:557-593``).
+* :func:`radtransfer` -- ``radtransfer_singlethread``,
+  ``src/meshoid/radiation/radtransfer.py:79-177`` (farthest first).
 * :meth:`OracleMeshoid.Reconstruct`, ``Slice``, ``InterpToGrid`` --
   ``src/meshoid/Meshoid.py:383-555`` (the nearest particle from :func:`knn`
   with k = 1, ties to the lowest index).
@@ -267,6 +269,46 @@ def grid_density(f, x, h, center, size, res=100, box_size=-1.0):
             gx, gy, gz = np.meshgrid(ranges[0] % res, ranges[1] % res, ranges[2] % res, indexing="ij")
             np.add.at(grid, (gx, gy, gz), f[i] * k / tot)
     return grid / (dx * dx * dx)
+
+
+def radtransfer(j, m, kappa, x, h, gridres, L, center=0, i0=0):
+    """radtransfer_singlethread (radiation/radtransfer.py:79-177): the
+    particles farthest first (largest z first); each changes the pixels of its
+    footprint (r <= max(h, dx)) in every band by tau = kappa kernel2d(r/h)
+    m / h^2, I <- exp(-tau) I - expm1(-tau) j / (4 pi kappa + 1e-100)."""
+    x = np.asarray(x, dtype=np.float64) - np.broadcast_to(np.asarray(center, dtype=np.float64), (3,))
+    j = np.asarray(j, dtype=np.float64)
+    if j.ndim == 1:
+        j = j[:, None]
+    nf = j.shape[1]
+    kappa = np.asarray(kappa, dtype=np.float64)
+    if kappa.ndim == 1:
+        kappa = np.repeat(kappa[:, None], nf, axis=1)
+    m = np.asarray(m, dtype=np.float64).reshape(-1)
+    h = np.asarray(h, dtype=np.float64).reshape(-1)
+    order = np.argsort(-x[:, 2], kind="stable")
+    dx = L / (gridres - 1)
+    intensity = np.zeros((gridres, gridres, nf)) + i0
+    for i in order:
+        xs = x[i] + L / 2
+        hs = max(h[i], dx)
+        if hs == 0 or m[i] == 0:
+            continue
+        src = j[i] / (4 * np.pi * kappa[i] + 1e-100)
+        mh2 = m[i] / hs**2
+        gx0, gx1 = max(int((xs[0] - hs) / dx + 1), 0), min(int((xs[0] + hs) / dx), gridres - 1)
+        gy0, gy1 = max(int((xs[1] - hs) / dx + 1), 0), min(int((xs[1] + hs) / dx), gridres - 1)
+        if gx1 < gx0 or gy1 < gy0:
+            continue
+        gx = np.arange(gx0, gx1 + 1)
+        gy = np.arange(gy0, gy1 + 1)
+        r = ((xs[0] - gx * dx) ** 2)[:, None] + ((xs[1] - gy * dx) ** 2)[None, :]
+        inside = r <= hs * hs
+        k = np.where(inside, kernel2d(np.minimum(np.sqrt(r) / hs, 1.0)), 0.0)
+        tau = kappa[i][None, None, :] * k[:, :, None] * mh2
+        blk = intensity[gx0:gx1 + 1, gy0:gy1 + 1]
+        blk[...] = np.where(inside[:, :, None], np.exp(-tau) * blk - np.expm1(-tau) * src[None, None, :], blk)
+    return intensity
 
 
 class OracleMeshoid:
