@@ -305,6 +305,23 @@ def surface_density_form(args, torch, dev, log, pos_d, h_d, n, timed):
     out["grid_density_res128"] = {"ms": kern.get("grid_density", dt * 1e3), "particles_per_s": n / (dt or 1e-30),
                                   "map_total_over_mass": float(g["g"].sum()) * dx**3}
     log("grid density res128", out["grid_density_res128"])
+    # radtransfer: one band, emissivity 1 per unit mass, opacity chosen so
+    # that the densest sightlines are optically thick
+    import meshoid_amd
+
+    xd = pos_d.to(torch.float64)
+    one = torch.ones((n, 1), dtype=torch.float64, device=dev)
+    kap = torch.full((n, 1), 3.0e4, dtype=torch.float64, device=dev)
+    hd = h_d.to(torch.float64)
+    img = {}
+
+    def step_rt():
+        img["i"] = meshoid_amd.radtransfer(one, m, kap, xd, hd, 128, 100.0, center=(50.0, 50.0, 50.0))
+
+    dt, kern, _ = timed(step_rt, 2, 1, 1)
+    out["radtransfer_res128"] = {"ms": kern.get("radtransfer", dt * 1e3), "particles_per_s": n / (dt or 1e-30),
+                                 "max_intensity": float(img["i"].max())}
+    log("radtransfer res128", out["radtransfer_res128"])
     return out
 
 

@@ -15,9 +15,10 @@
 // of consecutive particles: a slab computed from I = 0 gives (B_s, tau_s) and
 // acts as I <- exp(-tau_s) I + B_s.  Slabs x image tiles run in parallel, one
 // wavefront per (slab, tile) holding its tile in LDS and sweeping the slab's
-// particles in order (one lane per pixel of a footprint, so a wavefront's
-// LDS updates stay in program order); a last kernel composes the slabs per
-// pixel, farthest first, from the background i0.
+// particles in order, 64 loaded at a time and only those touching the tile
+// applied (one lane per pixel of a footprint, so a wavefront's LDS updates
+// stay in program order); a last kernel composes the slabs per pixel,
+// farthest first, from the background i0.
 //
 // The reference's multi-threaded driver composes its chunks with the largest
 // z nearest the observer while each chunk is swept with the largest z
@@ -101,35 +102,61 @@ __global__ __launch_bounds__(64) void k_rt_slab(RtArgs a) {
         T[c] = 0.0;
     }
     const int64_t k0 = (int64_t)slab * a.per, k1 = min(a.n, k0 + a.per);
-    for (int64_t k = k0; k < k1; k++) {
-        const double hraw = a.sh[k], m = a.sm[k];
-        const double hs = fmax(hraw, a.dx);
-        if (hs == 0.0 || m == 0.0) continue;
-        const double xs0 = a.sx[k], xs1 = a.sy[k];
-        // index range of the footprint, clipped to the grid and to the tile
-        const double l0 = (xs0 - hs) / a.dx + 1.0, h0 = (xs0 + hs) / a.dx;
-        const double l1 = (xs1 - hs) / a.dx + 1.0, h1 = (xs1 + hs) / a.dx;
-        if (!(h0 > -1.0) || !(h1 > -1.0) || !(l0 < (double)a.res) || !(l1 < (double)a.res)) continue;
-        const int gx0 = max(l0 > 0.0 ? (int)l0 : 0, tx0), gx1 = min(h0 >= a.res - 1 ? a.res - 1 : (int)h0, tx1);
-        const int gy0 = max(l1 > 0.0 ? (int)l1 : 0, ty0), gy1 = min(h1 >= a.res - 1 ? a.res - 1 : (int)h1, ty1);
-        if (gx0 > gx1 || gy0 > gy1) continue;
-        const double mh2 = m / (hs * hs), hs2 = hs * hs;
-        const int wy = gy1 - gy0 + 1, cells = (gx1 - gx0 + 1) * wy;
-        for (int c = lane; c < cells; c += 64) {
-            const int gx = gx0 + c / wy, gy = gy0 + c % wy;
-            double ddx = xs0 - gx * a.dx;
-            ddx *= ddx;
-            double ddy = xs1 - gy * a.dx;
-            ddy *= ddy;
-            const double r = ddx + ddy;
-            if (r > hs2) continue;
-            const double kern = rt_kernel2d(sqrt(r) / hs);
-            const int p = ((gx - tx0) * a.tw + (gy - ty0)) * a.nf;
-            for (int b = 0; b < a.nf; b++) {
-                const double tau = a.skap[k * a.nf + b] * kern * mh2;
-                const double fac1 = exp(-tau), fac2 = -expm1(-tau);
-                T[p + b] += tau;
-                I[p + b] = fac1 * I[p + b] + fac2 * a.ssrc[k * a.nf + b];
+    // 64 particles at a time: each lane loads one (coalesced) and tests it
+    // against the tile; the particles that touch the tile are then applied
+    // one after another in ascending order, their data broadcast from the
+    // loading lane
+    for (int64_t base = k0; base < k1; base += 64) {
+        const int64_t k = base + lane;
+        bool hit = false;
+        double xs0 = 0.0, xs1 = 0.0, hs = 0.0, m = 0.0, mh2 = 0.0, hinv = 0.0;
+        int gx0 = 0, gx1 = -1, gy0 = 0, gy1 = -1;
+        if (k < k1) {
+            m = a.sm[k];
+            hs = fmax(a.sh[k], a.dx);
+            xs0 = a.sx[k];
+            xs1 = a.sy[k];
+            // index range of the footprint, clipped to the grid and the tile
+            const double l0 = (xs0 - hs) / a.dx + 1.0, h0 = (xs0 + hs) / a.dx;
+            const double l1 = (xs1 - hs) / a.dx + 1.0, h1 = (xs1 + hs) / a.dx;
+            if (hs != 0.0 && m != 0.0 && h0 > -1.0 && h1 > -1.0 && l0 < (double)a.res && l1 < (double)a.res) {
+                gx0 = max(l0 > 0.0 ? (int)l0 : 0, tx0);
+                gx1 = min(h0 >= a.res - 1 ? a.res - 1 : (int)h0, tx1);
+                gy0 = max(l1 > 0.0 ? (int)l1 : 0, ty0);
+                gy1 = min(h1 >= a.res - 1 ? a.res - 1 : (int)h1, ty1);
+                hit = gx0 <= gx1 && gy0 <= gy1;
+                mh2 = m / (hs * hs);
+                hinv = 1.0 / hs;
+            }
+        }
+        uint64_t todo = __ballot(hit);
+        while (todo) {
+            const int src = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const int64_t kp = base + src;
+            const double pxs0 = __shfl(xs0, src), pxs1 = __shfl(xs1, src), phs = __shfl(hs, src);
+            const double pmh2 = __shfl(mh2, src), phinv = __shfl(hinv, src);
+            const int pgx0 = __shfl(gx0, src), pgx1 = __shfl(gx1, src), pgy0 = __shfl(gy0, src),
+                      pgy1 = __shfl(gy1, src);
+            const double hs2 = phs * phs;
+            const int wy = pgy1 - pgy0 + 1, cells = (pgx1 - pgx0 + 1) * wy;
+            for (int c = lane; c < cells; c += 64) {
+                const int gx = pgx0 + c / wy, gy = pgy0 + c % wy;
+                double ddx = pxs0 - gx * a.dx;
+                ddx *= ddx;
+                double ddy = pxs1 - gy * a.dx;
+                ddy *= ddy;
+                const double r = ddx + ddy;
+                if (r > hs2) continue;
+                const double kern = rt_kernel2d(sqrt(r) * phinv);
+                const int p = ((gx - tx0) * a.tw + (gy - ty0)) * a.nf;
+                for (int b = 0; b < a.nf; b++) {
+                    const double tau = a.skap[kp * a.nf + b] * kern * pmh2;
+                    // exp(-tau) = 1 + expm1(-tau): one transcendental per update
+                    const double em1 = expm1(-tau), fac1 = 1.0 + em1, fac2 = -em1;
+                    T[p + b] += tau;
+                    I[p + b] = fac1 * I[p + b] + fac2 * a.ssrc[kp * a.nf + b];
+                }
             }
         }
     }
@@ -168,13 +195,15 @@ int radtransfer(mhip_ctx *ctx, const double *j, const double *m, const double *k
     if (!(L > 0.0)) return fail(ctx, -1, "L must be positive");
     if (n > 0xffffffffll) return fail(ctx, -4, "more than 2^32 particles");
     const int64_t npix = (int64_t)res * res;
-    // tile side: LDS holds I and tau for tw^2 pixels and nf bands (<= 64 KB)
-    int tw = 32;
+    // tile side: LDS holds I and tau for tw^2 pixels and nf bands (<= 64 KB);
+    // small tiles spread a clump's pixels over many wavefronts
+    int tw = 16;
     while (tw > 4 && (int64_t)tw * tw * nf * 16 > 65536) tw /= 2;
     const int ntx = (res + tw - 1) / tw, ntile = ntx * ntx;
-    // slabs: enough to fill the GPU, slab maps within 1 GB
+    // slabs: about 16k (slab, tile) wavefronts, so that the sequential sweep
+    // of one wavefront stays short; slab maps within 1 GB
     const int64_t map_bytes = npix * nf * 16;
-    int64_t nslab = std::max<int64_t>(1, std::min<int64_t>(2048 / ntile + 1, (int64_t)(1ll << 30) / map_bytes));
+    int64_t nslab = std::max<int64_t>(1, std::min<int64_t>(16384 / ntile + 1, (int64_t)(1ll << 30) / map_bytes));
     nslab = std::min<int64_t>(nslab, std::max<int64_t>(1, n));
     const int64_t per = n > 0 ? (n + nslab - 1) / nslab : 1;
     const size_t cw = radix_counts_words(std::max<int64_t>(n, 1));

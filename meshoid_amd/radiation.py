@@ -43,7 +43,9 @@ def radtransfer(j, m, kappa, x, h, gridres, L, center=0, i0=0):
     i0a = np.asarray(i0, dtype=np.float64)
     i0d = None
     if np.any(i0a != 0):
-        i0d = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(i0a, (res, res, nf)))).to(dev)
+        # a C-ordered, writable copy: the library reads the buffer as (res, res, nf)
+        i0d = torch.from_numpy(np.array(np.broadcast_to(i0a, (res, res, nf)), dtype=np.float64, order="C"))
+        i0d = i0d.to(dev).contiguous()
     out = torch.empty((res, res, nf), dtype=torch.float64, device=dev)
     ctx = engine.context(dev)
     raise_for(lib.mhip_radtransfer(ctx, engine.ptr(jd), engine.ptr(md), engine.ptr(kd), engine.ptr(xd),
