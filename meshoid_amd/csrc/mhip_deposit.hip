@@ -26,7 +26,9 @@
 namespace mhip {
 
 constexpr int DEP_THREADS = 256;
-constexpr int64_t BIG_CELLS = 1024;           // footprint handled by one thread at most
+constexpr int64_t BIG_CELLS = 1024;           // footprint handled by one thread at most (LDS map)
+constexpr int64_t BIG_CELLS_HBM = 128;        // the same for HBM maps, where a long per-thread
+                                              // loop also holds its wavefront's atomics back
 constexpr int DEP_TILE = 128;                 // map tile side held in LDS (128 KB of float64)
 constexpr double KERNEL2D_NORM = 1.8189136353359467;  // 40 / (7 pi)
 
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_hbm(DepArgs a) {
         Foot fp;
         if (!footprint(a, i, fp)) continue;
         const int64_t cells = (int64_t)(fp.gx1 - fp.gx0 + 1) * (fp.gy1 - fp.gy0 + 1);
-        if (cells > BIG_CELLS) {
+        if (cells > BIG_CELLS_HBM) {
             a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;  // at most once per particle
             continue;
         }
