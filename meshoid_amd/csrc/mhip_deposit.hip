@@ -18,19 +18,31 @@
 // map in LDS (float64 LDS atomics, 128 KB) for an interleaved share of the
 // particles and adds it to the HBM map once at the end: the many particles
 // of a clump that land on the same few cells contend in LDS, not in L2.
-// Larger maps take float64 atomics in HBM.  Particles whose footprint exceeds
-// BIG_CELLS go to a second kernel that spreads one particle over a whole
-// block.
+// Larger maps (and every GridDensity volume) are binned: particles are
+// sorted by the brick (32^2 map cells, 8^3 volume cells) that holds the
+// first cell of their footprint, and each block takes a run of the sorted
+// particles; a brick's run of at least BIN_DENSE particles accumulates in an
+// LDS window (the brick plus the footprint reach above it, 4096 cells) that
+// is added to the HBM map once, shorter runs take float64 atomics in HBM.
+// Particles whose footprint exceeds the window (or BIG_CELLS) go to a kernel
+// that spreads one particle over a whole block.
 #include "mhip_internal.hpp"
 
 namespace mhip {
 
 constexpr int DEP_THREADS = 256;
-constexpr int64_t BIG_CELLS = 1024;           // footprint handled by one thread at most (LDS map)
-constexpr int64_t BIG_CELLS_HBM = 128;        // the same for HBM maps, where a long per-thread
-                                              // loop also holds its wavefront's atomics back
-constexpr int DEP_TILE = 128;                 // map tile side held in LDS (128 KB of float64)
+constexpr int64_t BIG_CELLS = 1024;           // footprint handled by one thread at most
+constexpr int DEP_TILE = 128;                 // map side held whole in LDS (128 KB of float64)
 constexpr double KERNEL2D_NORM = 1.8189136353359467;  // 40 / (7 pi)
+
+// binned deposition
+constexpr int BIN_CHUNK = 2048;   // sorted particles per block
+constexpr int BIN_DENSE = 32;     // a brick's run of this many particles in a block uses the window
+constexpr int WIN_CELLS = 4096;   // LDS window, float64 cells (32 KB)
+constexpr int BRICK2 = 32, WIN2 = 64;  // maps: brick side, window side (footprint span <= 33)
+constexpr int BRICK3 = 8, WIN3 = 16;   // volumes: brick side, window side (span <= 9)
+static_assert(WIN2 * WIN2 == WIN_CELLS && WIN3 * WIN3 * WIN3 == WIN_CELLS, "window size");
+static_assert(BIN_CHUNK <= 65535, "run offsets and lengths are packed in 16 bits");
 
 struct DepArgs {
     const void *f, *pos, *h;
@@ -42,6 +54,7 @@ struct DepArgs {
     int res, conservative;
     double *grid;           // (res, res), [gx][gy]
     uint32_t *big, *nbig;   // particles left to the block-per-particle kernel
+    int nb;                 // bricks per axis (binned path)
 };
 
 MHIP_DEV double kernel2d(double q) {
@@ -157,31 +170,177 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_tiles(DepArgs a, int tw
     }
 }
 
-// maps larger than one tile: every thread deposits its particles straight
-// into the HBM map (float64 atomics), particles interleaved over the grid
-__global__ __launch_bounds__(DEP_THREADS) void k_deposit_hbm(DepArgs a) {
-    for (int64_t i = blockIdx.x * (int64_t)DEP_THREADS + threadIdx.x; i < a.n;
-         i += (int64_t)gridDim.x * DEP_THREADS) {
+// conservative variant: the particle's total weight over its (clipped)
+// footprint, by which each weight is divided; 1 for the plain variant
+MHIP_DEV double foot_total(const DepArgs &a, const Foot &fp) {
+    if (!a.conservative) return 1.0;
+    double tot = 0.0;
+    for (int gx = fp.gx0; gx <= fp.gx1; gx++)
+        for (int gy = fp.gy0; gy <= fp.gy1; gy++) tot += cell_weight(a, fp, gx, gy);
+    return tot;
+}
+
+// ---------------------------------------------------------------------------
+// Binned deposition.  A policy P gives, for its kind of grid:
+//   key(a, i)            the brick of particle i's first footprint cell, or
+//                        skip(a) when it has nothing to deposit here (no
+//                        footprint, or listed for the block-per-particle kernel)
+//   direct(a, i)         deposit particle i with float64 atomics in HBM
+//   window(a, i, k, w)   deposit particle i (brick k) into the LDS window w
+//   flush(a, k, c, v)    add window cell c of brick k (value v) to the map
+// ---------------------------------------------------------------------------
+struct Map2 {
+    using Args = DepArgs;
+    static MHIP_DEV uint32_t skip(const DepArgs &a) { return (uint32_t)a.nb * (uint32_t)a.nb; }
+    static MHIP_DEV uint32_t key(const DepArgs &a, int64_t i) {
         Foot fp;
-        if (!footprint(a, i, fp)) continue;
-        const int64_t cells = (int64_t)(fp.gx1 - fp.gx0 + 1) * (fp.gy1 - fp.gy0 + 1);
-        if (cells > BIG_CELLS_HBM) {
-            a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;  // at most once per particle
-            continue;
+        if (!footprint(a, i, fp)) return skip(a);
+        const int sx = fp.gx1 - fp.gx0 + 1, sy = fp.gy1 - fp.gy0 + 1;
+        if (sx > WIN2 - BRICK2 + 1 || sy > WIN2 - BRICK2 + 1 || (int64_t)sx * sy > BIG_CELLS) {
+            a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;
+            return skip(a);
         }
-        double tot = 1.0;
-        if (a.conservative) {
-            tot = 0.0;
-            for (int gx = fp.gx0; gx <= fp.gx1; gx++)
-                for (int gy = fp.gy0; gy <= fp.gy1; gy++) tot += cell_weight(a, fp, gx, gy);
-            if (tot == 0.0) continue;
-        }
+        return (uint32_t)(fp.gx0 / BRICK2) * (uint32_t)a.nb + (uint32_t)(fp.gy0 / BRICK2);
+    }
+    static MHIP_DEV void direct(const DepArgs &a, uint32_t i) {
+        Foot fp;
+        footprint(a, i, fp);  // keyed, so it has a footprint
+        const double tot = foot_total(a, fp);
+        if (tot == 0.0) return;
         for (int gx = fp.gx0; gx <= fp.gx1; gx++)
             for (int gy = fp.gy0; gy <= fp.gy1; gy++) {
                 const double k = cell_weight(a, fp, gx, gy);
                 if (k != 0.0) atomicAdd(&a.grid[(int64_t)gx * a.res + gy], a.conservative ? k * fp.w / tot : k * fp.w);
             }
     }
+    // the window holds cells [ox, ox + 64) x [oy, oy + 64) of the brick's
+    // origin (ox, oy); a keyed footprint starts inside the brick and spans at
+    // most 33 cells per axis, so it stays inside
+    static MHIP_DEV void window(const DepArgs &a, uint32_t i, uint32_t key, double *win) {
+        Foot fp;
+        footprint(a, i, fp);
+        const double tot = foot_total(a, fp);
+        if (tot == 0.0) return;
+        const int ox = (int)(key / (uint32_t)a.nb) * BRICK2, oy = (int)(key % (uint32_t)a.nb) * BRICK2;
+        for (int gx = fp.gx0; gx <= fp.gx1; gx++)
+            for (int gy = fp.gy0; gy <= fp.gy1; gy++) {
+                const double k = cell_weight(a, fp, gx, gy);
+                if (k != 0.0) atomicAdd(&win[(gx - ox) * WIN2 + (gy - oy)], a.conservative ? k * fp.w / tot : k * fp.w);
+            }
+    }
+    static MHIP_DEV void flush(const DepArgs &a, uint32_t key, int c, double v) {
+        const int gx = (int)(key / (uint32_t)a.nb) * BRICK2 + c / WIN2;
+        const int gy = (int)(key % (uint32_t)a.nb) * BRICK2 + c % WIN2;
+        if (gx < a.res && gy < a.res) atomicAdd(&a.grid[(int64_t)gx * a.res + gy], v);
+    }
+};
+
+template <class P>
+__global__ __launch_bounds__(256) void k_bin_keys(typename P::Args a, uint64_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ vals) {
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i >= a.n) return;
+    keys[i] = P::key(a, i);
+    vals[i] = (uint32_t)i;
+}
+
+MHIP_DEV int first_not_below(const uint32_t *s, int m, uint32_t k) {
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s[mid] < k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+MHIP_DEV int first_above(const uint32_t *s, int m, uint32_t k) {
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s[mid] <= k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// block b: sorted particles [b BIN_CHUNK, (b + 1) BIN_CHUNK).  Their bricks'
+// runs (lengths from binary searches of the block's keys in LDS) shorter
+// than BIN_DENSE are deposited straight into HBM; each longer run is
+// deposited into the LDS window by the whole block and added to the map
+// once (cells still zero are skipped), the window cleared as it is read
+template <class P>
+__global__ __launch_bounds__(DEP_THREADS) void k_bin_deposit(typename P::Args a, const uint64_t *__restrict__ keys,
+                                                             const uint32_t *__restrict__ vals) {
+    __shared__ double win[WIN_CELLS];
+    __shared__ uint32_t skey[BIN_CHUNK];
+    __shared__ uint32_t runs[BIN_CHUNK / BIN_DENSE];  // offset | length << 16
+    __shared__ uint32_t s_nrun;
+    const int64_t cs = (int64_t)blockIdx.x * BIN_CHUNK;
+    const int m = (int)min<int64_t>(BIN_CHUNK, a.n - cs);
+    const uint32_t skip = P::skip(a);
+    for (int c = threadIdx.x; c < WIN_CELLS; c += DEP_THREADS) win[c] = 0.0;
+    for (int e = threadIdx.x; e < m; e += DEP_THREADS) skey[e] = (uint32_t)keys[cs + e];
+    if (threadIdx.x == 0) s_nrun = 0u;
+    __syncthreads();
+    for (int e = threadIdx.x; e < m; e += DEP_THREADS) {
+        const uint32_t k = skey[e];
+        if (k == skip) continue;
+        const int lo = first_not_below(skey, m, k), hi = first_above(skey, m, k);
+        if (hi - lo >= BIN_DENSE) {
+            // at most BIN_CHUNK / BIN_DENSE runs this long
+            if (e == lo) runs[atomicAdd(&s_nrun, 1u)] = (uint32_t)lo | (uint32_t)(hi - lo) << 16;
+        } else {
+            P::direct(a, vals[cs + e]);
+        }
+    }
+    __syncthreads();
+    const uint32_t nrun = s_nrun;
+    for (uint32_t r = 0; r < nrun; r++) {
+        const int lo = (int)(runs[r] & 0xffffu), len = (int)(runs[r] >> 16);
+        const uint32_t k = skey[lo];
+        for (int e = threadIdx.x; e < len; e += DEP_THREADS) P::window(a, vals[cs + lo + e], k, win);
+        __syncthreads();
+        for (int c = threadIdx.x; c < WIN_CELLS; c += DEP_THREADS) {
+            const double v = win[c];
+            if (v != 0.0) {
+                P::flush(a, k, c, v);
+                win[c] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// bits of a sort key in [0, skip]
+static int key_bits(uint32_t skip) { return 64 - __builtin_clzll((unsigned long long)skip | 1ull); }
+
+// the binned path: keys, sort, runs; the block-per-particle list in big/nbig
+// (cleared beforehand).  Scratch: keys 16 n, values 8 n, radix counts.
+template <class P>
+int bin_deposit(mhip_ctx *ctx, typename P::Args a, uint32_t skip, char *sc) {
+    const int64_t n = a.n;
+    if (n <= 0) return 0;
+    uint64_t *k0 = (uint64_t *)sc, *k1 = k0 + n;
+    uint32_t *v0 = (uint32_t *)(k1 + n), *v1 = v0 + n, *counts = v1 + n;
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_bin_keys<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, k0, v0);
+    uint64_t *ks;
+    uint32_t *vs;
+    int rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, key_bits(skip), &ks, &vs);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bin_deposit<P>, dim3((unsigned)((n + BIN_CHUNK - 1) / BIN_CHUNK)), dim3(DEP_THREADS), 0, s,
+                       a, ks, vs);
+    return 0;
+}
+
+// scratch bytes of bin_deposit plus the block-per-particle list (n + 1 words)
+static size_t bin_scratch_bytes(int64_t n) {
+    return (size_t)n * 24 + radix_counts_words(std::max<int64_t>(n, 1)) * 4 + ((size_t)n + 1) * 4 + 256;
 }
 
 // one block per listed particle (footprints above BIG_CELLS), HBM atomics
@@ -251,6 +410,7 @@ struct Dep3Args {
     int res, periodic;
     double *grid;  // (res, res, res)
     uint32_t *big, *nbig;
+    int nb;  // bricks per axis
 };
 
 struct Foot3 {
@@ -317,21 +477,49 @@ MHIP_DEV int64_t cell3(const Dep3Args &a, int gx, int gy, int gz) {
     return ((int64_t)gx * a.res + gy) * a.res + gz;
 }
 
-__global__ __launch_bounds__(DEP_THREADS) void k_density3d(Dep3Args a) {
-    for (int64_t i = blockIdx.x * (int64_t)DEP_THREADS + threadIdx.x; i < a.n;
-         i += (int64_t)gridDim.x * DEP_THREADS) {
+MHIP_DEV double foot3_total(const Dep3Args &a, const Foot3 &fp) {
+    double tot = 0.0;
+    for (int gx = fp.g0[0]; gx <= fp.g1[0]; gx++)
+        for (int gy = fp.g0[1]; gy <= fp.g1[1]; gy++)
+            for (int gz = fp.g0[2]; gz <= fp.g1[2]; gz++) tot += weight3(a, fp, gx, gy, gz);
+    return tot;
+}
+
+// first footprint index on axis c, wrapped into the grid when periodic
+MHIP_DEV int wrapped_first(const Dep3Args &a, const Foot3 &fp, int c) {
+    return a.periodic ? ((fp.g0[c] % a.res) + a.res) % a.res : fp.g0[c];
+}
+
+// binned volumes: the brick of the (wrapped) first footprint cell; the
+// window holds 16^3 cells from the brick's origin, a keyed footprint spans at
+// most 9 cells per axis from inside the brick.  Periodic footprints keep
+// their unwrapped extent in the window and wrap when it is added to the map.
+struct Vol3 {
+    using Args = Dep3Args;
+    static MHIP_DEV uint32_t skip(const Dep3Args &a) { return (uint32_t)a.nb * (uint32_t)a.nb * (uint32_t)a.nb; }
+    static MHIP_DEV uint32_t key(const Dep3Args &a, int64_t i) {
         Foot3 fp;
-        if (!footprint3(a, i, fp)) continue;
-        const int64_t cells = (int64_t)(fp.g1[0] - fp.g0[0] + 1) * (fp.g1[1] - fp.g0[1] + 1) * (fp.g1[2] - fp.g0[2] + 1);
-        if (cells > BIG_CELLS_3D) {
-            a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;  // at most once per particle
-            continue;
+        if (!footprint3(a, i, fp)) return skip(a);
+        int64_t cells = 1;
+        bool fits = true;
+        for (int c = 0; c < 3; c++) {
+            const int64_t sp = (int64_t)fp.g1[c] - fp.g0[c] + 1;
+            fits = fits && sp <= WIN3 - BRICK3 + 1;
+            cells *= fits ? sp : 1;
         }
-        double tot = 0.0;
-        for (int gx = fp.g0[0]; gx <= fp.g1[0]; gx++)
-            for (int gy = fp.g0[1]; gy <= fp.g1[1]; gy++)
-                for (int gz = fp.g0[2]; gz <= fp.g1[2]; gz++) tot += weight3(a, fp, gx, gy, gz);
-        if (!(tot > 0.0)) continue;
+        if (!fits || cells > BIG_CELLS_3D) {
+            a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;
+            return skip(a);
+        }
+        const uint32_t nb = (uint32_t)a.nb;
+        return ((uint32_t)(wrapped_first(a, fp, 0) / BRICK3) * nb + (uint32_t)(wrapped_first(a, fp, 1) / BRICK3)) * nb +
+               (uint32_t)(wrapped_first(a, fp, 2) / BRICK3);
+    }
+    static MHIP_DEV void direct(const Dep3Args &a, uint32_t i) {
+        Foot3 fp;
+        footprint3(a, i, fp);  // keyed, so it has a footprint
+        const double tot = foot3_total(a, fp);
+        if (!(tot > 0.0)) return;
         for (int gx = fp.g0[0]; gx <= fp.g1[0]; gx++)
             for (int gy = fp.g0[1]; gy <= fp.g1[1]; gy++)
                 for (int gz = fp.g0[2]; gz <= fp.g1[2]; gz++) {
@@ -339,7 +527,36 @@ __global__ __launch_bounds__(DEP_THREADS) void k_density3d(Dep3Args a) {
                     if (k != 0.0) atomicAdd(&a.grid[cell3(a, gx, gy, gz)], fp.f * k / tot);
                 }
     }
-}
+    static MHIP_DEV void window(const Dep3Args &a, uint32_t i, uint32_t key, double *win) {
+        Foot3 fp;
+        footprint3(a, i, fp);
+        const double tot = foot3_total(a, fp);
+        if (!(tot > 0.0)) return;
+        // window index of the footprint's first cell on each axis (0..7)
+        const uint32_t nb = (uint32_t)a.nb;
+        const int b[3] = {(int)(key / (nb * nb)), (int)(key / nb % nb), (int)(key % nb)};
+        int s[3];
+        for (int c = 0; c < 3; c++) s[c] = wrapped_first(a, fp, c) - b[c] * BRICK3;
+        for (int gx = fp.g0[0]; gx <= fp.g1[0]; gx++)
+            for (int gy = fp.g0[1]; gy <= fp.g1[1]; gy++)
+                for (int gz = fp.g0[2]; gz <= fp.g1[2]; gz++) {
+                    const double k = weight3(a, fp, gx, gy, gz);
+                    if (k != 0.0) {
+                        const int c = ((gx - fp.g0[0] + s[0]) * WIN3 + (gy - fp.g0[1] + s[1])) * WIN3 +
+                                      (gz - fp.g0[2] + s[2]);
+                        atomicAdd(&win[c], fp.f * k / tot);
+                    }
+                }
+    }
+    static MHIP_DEV void flush(const Dep3Args &a, uint32_t key, int c, double v) {
+        const uint32_t nb = (uint32_t)a.nb;
+        const int gx = (int)(key / (nb * nb)) * BRICK3 + c / (WIN3 * WIN3);
+        const int gy = (int)(key / nb % nb) * BRICK3 + c / WIN3 % WIN3;
+        const int gz = (int)(key % nb) * BRICK3 + c % WIN3;
+        if (!a.periodic && (gx >= a.res || gy >= a.res || gz >= a.res)) return;
+        atomicAdd(&a.grid[cell3(a, gx, gy, gz)], v);
+    }
+};
 
 // one block per listed particle (footprints above BIG_CELLS_3D)
 __global__ __launch_bounds__(DEP_THREADS) void k_density3d_big(Dep3Args a) {
@@ -396,11 +613,13 @@ int grid_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int p
     a.res = res;
     a.periodic = periodic ? 1 : 0;
     a.grid = grid;
+    a.nb = (res + BRICK3 - 1) / BRICK3;
     int rc = 0;
-    uint32_t *sc = (uint32_t *)scratch(ctx, (size_t)(n + 1) * 4, &rc);
+    char *sc = (char *)scratch(ctx, bin_scratch_bytes(n), &rc);
     if (!sc) return rc;
-    a.nbig = sc;
-    a.big = sc + 1;
+    // the sort buffers first (8-byte keys), the block-per-particle list last
+    a.nbig = (uint32_t *)(sc + (size_t)n * 24 + radix_counts_words(std::max<int64_t>(n, 1)) * 4);
+    a.big = a.nbig + 1;
     hipStream_t s = ctx->stream;
     const int64_t ncell = (int64_t)res * res * res;
     ProfRegion pr(ctx, "grid_density");
@@ -408,8 +627,7 @@ int grid_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int p
                        grid, ncell, 0.0);
     hipLaunchKernelGGL(k_u32_zero, dim3(1), dim3(64), 0, s, a.nbig);
     if (n > 0) {
-        hipLaunchKernelGGL(k_density3d, dim3((unsigned)std::min<int64_t>(8192, (n + 255) / 256)), dim3(DEP_THREADS),
-                           0, s, a);
+        if ((rc = bin_deposit<Vol3>(ctx, a, (uint32_t)(a.nb * a.nb * a.nb), sc))) return rc;
         hipLaunchKernelGGL(k_density3d_big, dim3(2048), dim3(DEP_THREADS), 0, s, a);
     }
     const double inv = 1.0 / (a.dx * a.dx * a.dx);
@@ -422,7 +640,7 @@ int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, in
                     const void *h, int h_f64, double smooth, double hmin, double cx, double cy, double size, int res,
                     int conservative, double *grid) {
     if (n < 0 || dim < 2 || dim > 3) return fail(ctx, -1, "surface density needs (n, 2) or (n, 3) positions");
-    if (res < 2) return fail(ctx, -1, "res must be at least 2");
+    if (res < 2 || res > (1 << 20)) return fail(ctx, -1, "res must be in [2, 2^20]");
     if (!(size > 0.0)) return fail(ctx, -1, "size must be positive");
     if (n > 0xffffffffll) return fail(ctx, -4, "more than 2^32 particles");
     DepArgs a;
@@ -442,11 +660,12 @@ int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, in
     a.res = res;
     a.conservative = conservative ? 1 : 0;
     a.grid = grid;
+    a.nb = (res + BRICK2 - 1) / BRICK2;
     int rc = 0;
-    uint32_t *sc = (uint32_t *)scratch(ctx, (size_t)(n + 1) * 4, &rc);
+    char *sc = (char *)scratch(ctx, bin_scratch_bytes(n), &rc);
     if (!sc) return rc;
-    a.nbig = sc;
-    a.big = sc + 1;
+    a.nbig = (uint32_t *)(sc + (size_t)n * 24 + radix_counts_words(std::max<int64_t>(n, 1)) * 4);
+    a.big = a.nbig + 1;
     hipStream_t s = ctx->stream;
     const int64_t ncell = (int64_t)res * res;
     ProfRegion pr(ctx, "surface_density");
@@ -468,11 +687,9 @@ int surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, in
             hipLaunchKernelGGL(k_deposit_tiles, dim3((unsigned)nchunk), dim3(DEP_THREADS), lds, s, a, res, 1,
                                (int)nchunk);
         } else {
-            // per-tile LDS copies would re-read every particle per tile and
-            // pay each wavefront's largest in-tile footprint once per tile
-            // (measured 3x slower at 1024^2): straight HBM atomics instead
-            hipLaunchKernelGGL(k_deposit_hbm, dim3((unsigned)std::min<int64_t>(8192, (n + 255) / 256)),
-                               dim3(DEP_THREADS), 0, s, a);
+            // binned by 32^2 bricks (per-tile LDS copies of the map would
+            // re-read every particle per tile)
+            if ((rc = bin_deposit<Map2>(ctx, a, (uint32_t)a.nb * (uint32_t)a.nb, sc))) return rc;
         }
         hipLaunchKernelGGL(k_deposit_big, dim3(2048), dim3(DEP_THREADS), 0, s, a);
     }

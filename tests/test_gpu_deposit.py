@@ -1,7 +1,7 @@
 """SurfaceDensity / GridSurfaceDensity (src/meshoid/Meshoid.py:595-643,
 src/meshoid/grid_deposition.py:123-295) on the GPU against the oracle
 (tests/test_deposit_oracle.py pins it): both modes, the LDS-resident grid
-(res <= 128) and the HBM-atomic grid (res > 128), particles off the map,
+(res <= 128) and the binned LDS-window path (res > 128), particles off the map,
 footprints large enough for the block-per-particle kernel, float32 and
 float64 inputs, 2D and 3D positions.  Float64 sums in another order: maps
 agree to 1e-11 of their maximum."""
@@ -136,3 +136,31 @@ def test_grid_surface_density_module_function(oracle):
     for cons in (False, True):
         g = meshoid_amd.GridSurfaceDensity(f, pos, h, [5.0, 5.0], 10.0, res=64, conservative=cons)
         _close(g, oracle.grid_surface_density(f, pos, h, [5.0, 5.0], 10.0, 64, cons), f"module cons {cons}")
+
+
+def test_binned_deposition_clustered(oracle):
+    """The binned path on clustered particles: bricks whose runs span several
+    blocks and go through the LDS window, sparse background bricks on HBM
+    atomics, footprints reaching the window's far edge, periodic wrap."""
+    torch = _torch()
+    from meshoid_amd import engine, synthetic
+
+    pos = synthetic.clustered(100000, seed=7, nclump=16, box=10.0, background=0.1).astype(np.float64)
+    rng = np.random.default_rng(1)
+    h = np.exp(rng.normal(-3.0, 0.8, len(pos)))
+    h[rng.random(len(pos)) < 0.01] = 0.4  # spans of ~33 map cells at res 400, ~6 volume cells at res 64
+    f = rng.random(len(pos)) + 0.1
+    td = lambda a: torch.from_numpy(a).cuda()
+    c2 = np.array([5.0, 5.0])
+    for cons in (False, True):
+        say("binned surface density res 400 conservative", cons)
+        g = engine.surface_density(td(f), td(pos), td(h), c2, 10.0, 400, 1.0, cons)
+        go = oracle.grid_surface_density(f, pos, np.clip(h, 2 * 10.0 / 400, 1e100), c2, 10.0, 400, cons)
+        _close(g.cpu().numpy(), go, f"binned res 400 cons {cons}")
+    c3 = np.array([5.0, 5.0, 5.0])
+    dx = 10.0 / 63
+    for periodic in (False, True):
+        say("binned grid density res 64 periodic", periodic)
+        g = engine.grid_density(td(f), td(pos), td(h), c3, 10.0, 64, periodic, dx)
+        go = oracle.grid_density(f, pos, np.maximum(h, dx), c3, 10.0, 64, 10.0 if periodic else -1.0)
+        _close(g.cpu().numpy(), go, f"binned grid density periodic {periodic}")
