@@ -29,6 +29,12 @@ int radix_sort_pairs(mhip_ctx *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, ui
                      uint32_t *counts, int64_t n, int bits, uint64_t **kout, uint32_t **vout);
 size_t radix_counts_words(int64_t n);
 
+// exclusive prefix sum of in[0, n) into out (uint32: the total must stay below
+// 2^32); work holds scan_work_words(n) words.  Returns the device address of
+// the total (inside work).
+uint32_t *exclusive_scan_u32(mhip_ctx *ctx, const uint32_t *in, int64_t n, uint32_t *out, uint32_t *work);
+size_t scan_work_words(int64_t n);
+
 // spatial index build (bbox -> keys -> sort -> reorder -> boxes).  With rec
 // (device, 12 floats: bbox lo/hi, NaN flag, pad, frame) the build neither
 // synchronises nor checks: the record is left for the caller to validate

@@ -108,6 +108,27 @@ __global__ __launch_bounds__(256) void k_inverse(const int64_t *__restrict__ nbr
     inv[i] = (j >= 0 && j < n) ? (int64_t)(tile_sum[j / SCAN_TILE] + rank[j]) : -1;
 }
 
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t *__restrict__ out, const uint32_t *__restrict__ tile_sum,
+                                                  int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i < n) out[i] += tile_sum[i / SCAN_TILE];
+}
+
+size_t scan_work_words(int64_t n) { return (size_t)((n + SCAN_TILE - 1) / SCAN_TILE) + 1; }
+
+uint32_t *exclusive_scan_u32(mhip_ctx *ctx, const uint32_t *in, int64_t n, uint32_t *out, uint32_t *work) {
+    const int64_t ntile = (n + SCAN_TILE - 1) / SCAN_TILE;
+    hipStream_t s = ctx->stream;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntile), dim3(SCAN_THREADS), 0, s, in, n, out, work);
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(SCAN_THREADS), 0, s, work, ntile);
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, work, n);
+    } else {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, work, (int64_t)1);
+    }
+    return work + ntile;
+}
+
 int unique_inverse(mhip_ctx *ctx, const int64_t *nbr, int64_t nt, int64_t n, int64_t *uniq, int64_t *inv,
                    int64_t *nuniq) {
     if (nt < 0 || n < 1) return fail(ctx, -1, "bad unique shape");

@@ -1,8 +1,9 @@
 """radtransfer (src/meshoid/radiation/radtransfer.py:11-177) on the GPU
 against the oracle's sequential farthest-first sweep
-(tests/test_radtransfer_oracle.py pins it): slabs composed exactly, one or
-several bands, scalar / per-band / per-pixel backgrounds, optically thick
-and thin particles, particles off the image."""
+(tests/test_radtransfer_oracle.py pins it): per-tile segments composed
+exactly, one or several bands, scalar / per-band / per-pixel backgrounds,
+optically thick and thin particles, particles off the image, clumps whose
+tiles span many chunks, no particle on the image."""
 import sys
 
 import numpy as np
@@ -48,3 +49,42 @@ def test_radtransfer_matches_oracle(oracle, n, nf, res, i0):
     err = np.abs(img - want).max() / np.abs(want).max()
     say("  max rel to image max", err)
     assert err <= 1e-11
+
+
+def test_radtransfer_clustered_many_bands(oracle):
+    """Clumps put thousands of particles on a few tiles, so one tile's records
+    run over many chunks (composed per pixel in order); 20 bands shrink the
+    tiles to 8 x 8 sightlines."""
+    _torch()
+    import meshoid_amd
+    from meshoid_amd import synthetic
+
+    n, nf, res = 30000, 20, 64
+    x = synthetic.clustered(n, seed=9, nclump=5, box=10.0, background=0.05).astype(np.float64)
+    rng = np.random.default_rng(9)
+    m = rng.random(n) / n
+    h = np.exp(rng.normal(-2.0, 1.0, n))
+    j = rng.random((n, nf))
+    kap = np.exp(rng.normal(3.0, 2.0, (n, nf)))
+    say("radtransfer clustered n", n, "bands", nf, "res", res)
+    img = meshoid_amd.radtransfer(j, m, kap, x, h, res, 10.0, center=[5.0, 5.0, 5.0], i0=0.5)
+    want = oracle.radtransfer(j, m, kap, x, h, res, 10.0, center=[5.0, 5.0, 5.0], i0=0.5)
+    err = np.abs(img - want).max() / np.abs(want).max()
+    say("  max rel to image max", err)
+    assert err <= 1e-11
+
+
+def test_radtransfer_no_particles_on_the_image():
+    """No particles, or none whose footprint reaches a sightline: the image
+    is the background."""
+    _torch()
+    import meshoid_amd
+
+    i0 = np.random.default_rng(2).random((16, 16, 2))
+    e = np.zeros((0, 2))
+    img = meshoid_amd.radtransfer(e, np.zeros(0), e, np.zeros((0, 3)), np.zeros(0), 16, 4.0, i0=i0)
+    assert np.array_equal(img, i0)
+    x = np.full((50, 3), 100.0)
+    one = np.ones((50, 2))
+    img = meshoid_amd.radtransfer(one, np.ones(50), one, x, np.full(50, 0.1), 16, 4.0, i0=i0)
+    assert np.array_equal(img, i0)
