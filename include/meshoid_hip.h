@@ -210,6 +210,7 @@ int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, in
  * (SurfaceDensity: hmin = 2 size / res; GridSurfaceDensity itself: 1, 0);
  * conservative = 1: each particle's weights normalised to f, grid / dx^2
  * (GridSurfaceDensity_conservative_core).  Accumulated in float64.
+ * res in [2, 2^20].  Uses context scratch of about 28 n bytes.
  */
 int mhip_surface_density(mhip_ctx *ctx, const void *f, int f_f64, const void *pos, int pos_f64, int64_t n, int dim,
                          const void *h, int h_f64, double smooth_fac, double hmin, double cx, double cy, double size,
