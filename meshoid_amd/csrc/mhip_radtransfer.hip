@@ -33,6 +33,7 @@
 namespace mhip {
 
 constexpr double FOUR_PI = 12.566370614359172;
+constexpr int64_t RT_CHUNK = 1024;  // records per wavefront, at least
 constexpr double RT_KERNEL2D_NORM = 1.8189136353359467;  // 40 / (7 pi)
 
 MHIP_DEV double rt_kernel2d(double q) {
@@ -332,12 +333,12 @@ int radtransfer(mhip_ctx *ctx, const double *j, const double *m, const double *k
         if ((rc = check_hip(ctx, hipStreamSynchronize(s), "radtransfer count"))) return rc;
         if (nrec > 0xffffffffull) return fail(ctx, -4, "more than 2^32 (particle, tile) pairs");
     }
-    // records per chunk: at least 256, and the segment maps within 1 GB (or
-    // twice one per tile)
+    // records per chunk: at least RT_CHUNK, and the segment maps within 1 GB
+    // (or twice one per tile)
     const int64_t tcells = (int64_t)tw * tw * nf;
     const int64_t slot_bytes = tcells * 16;
     const int64_t avail = std::max<int64_t>(2 * (int64_t)ntile, (int64_t)(1ll << 30) / slot_bytes);
-    const int64_t per = std::max<int64_t>(256, ((int64_t)nrec + (avail - ntile) - 1) / (avail - ntile));
+    const int64_t per = std::max<int64_t>(RT_CHUNK, ((int64_t)nrec + (avail - ntile) - 1) / (avail - ntile));
     const int64_t nchunk = ((int64_t)nrec + per - 1) / per;
     const int64_t nslot = nchunk + ntile;
     const int64_t nr = (int64_t)nrec;
