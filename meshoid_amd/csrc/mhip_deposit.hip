@@ -43,6 +43,11 @@ constexpr int BRICK2 = 32, WIN2 = 64;  // maps: brick side, window side (footpri
 constexpr int BRICK3 = 8, WIN3 = 16;   // volumes: brick side, window side (span <= 9)
 static_assert(WIN2 * WIN2 == WIN_CELLS && WIN3 * WIN3 * WIN3 == WIN_CELLS, "window size");
 static_assert(BIN_CHUNK <= 65535, "run offsets and lengths are packed in 16 bits");
+// sort keys: brick << SIZE_BITS | (SIZE_MAX_RANK - footprint cells), so a
+// brick's particles come largest footprint first and a wavefront's 64
+// footprints are alike (its lanes finish together)
+constexpr int SIZE_BITS = 11;
+constexpr uint32_t SIZE_MAX_RANK = (1u << SIZE_BITS) - 1;
 
 struct DepArgs {
     const void *f, *pos, *h;
@@ -182,9 +187,10 @@ MHIP_DEV double foot_total(const DepArgs &a, const Foot &fp) {
 
 // ---------------------------------------------------------------------------
 // Binned deposition.  A policy P gives, for its kind of grid:
-//   key(a, i)            the brick of particle i's first footprint cell, or
-//                        skip(a) when it has nothing to deposit here (no
-//                        footprint, or listed for the block-per-particle kernel)
+//   key(a, i, cells)     the brick of particle i's first footprint cell (and
+//                        the footprint's cells), or skip(a) when it has
+//                        nothing to deposit here (no footprint, or listed for
+//                        the block-per-particle kernel)
 //   direct(a, i)         deposit particle i with float64 atomics in HBM
 //   window(a, i, k, w)   deposit particle i (brick k) into the LDS window w
 //   flush(a, k, c, v)    add window cell c of brick k (value v) to the map
@@ -192,14 +198,16 @@ MHIP_DEV double foot_total(const DepArgs &a, const Foot &fp) {
 struct Map2 {
     using Args = DepArgs;
     static MHIP_DEV uint32_t skip(const DepArgs &a) { return (uint32_t)a.nb * (uint32_t)a.nb; }
-    static MHIP_DEV uint32_t key(const DepArgs &a, int64_t i) {
+    static MHIP_DEV uint32_t key(const DepArgs &a, int64_t i, uint32_t &cells) {
         Foot fp;
+        cells = 0;
         if (!footprint(a, i, fp)) return skip(a);
         const int sx = fp.gx1 - fp.gx0 + 1, sy = fp.gy1 - fp.gy0 + 1;
         if (sx > WIN2 - BRICK2 + 1 || sy > WIN2 - BRICK2 + 1 || (int64_t)sx * sy > BIG_CELLS) {
             a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;
             return skip(a);
         }
+        cells = (uint32_t)(sx * sy);
         return (uint32_t)(fp.gx0 / BRICK2) * (uint32_t)a.nb + (uint32_t)(fp.gy0 / BRICK2);
     }
     static MHIP_DEV void direct(const DepArgs &a, uint32_t i) {
@@ -240,7 +248,9 @@ __global__ __launch_bounds__(256) void k_bin_keys(typename P::Args a, uint64_t *
                                                   uint32_t *__restrict__ vals) {
     const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (i >= a.n) return;
-    keys[i] = P::key(a, i);
+    uint32_t cells;
+    const uint32_t b = P::key(a, i, cells);
+    keys[i] = (uint64_t)b << SIZE_BITS | (SIZE_MAX_RANK - min(cells, SIZE_MAX_RANK));
     vals[i] = (uint32_t)i;
 }
 
@@ -284,7 +294,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_bin_deposit(typename P::Args a,
     const int m = (int)min<int64_t>(BIN_CHUNK, a.n - cs);
     const uint32_t skip = P::skip(a);
     for (int c = threadIdx.x; c < WIN_CELLS; c += DEP_THREADS) win[c] = 0.0;
-    for (int e = threadIdx.x; e < m; e += DEP_THREADS) skey[e] = (uint32_t)keys[cs + e];
+    for (int e = threadIdx.x; e < m; e += DEP_THREADS) skey[e] = (uint32_t)(keys[cs + e] >> SIZE_BITS);
     if (threadIdx.x == 0) s_nrun = 0u;
     __syncthreads();
     for (int e = threadIdx.x; e < m; e += DEP_THREADS) {
@@ -331,7 +341,7 @@ int bin_deposit(mhip_ctx *ctx, typename P::Args a, uint32_t skip, char *sc) {
     hipLaunchKernelGGL(k_bin_keys<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, k0, v0);
     uint64_t *ks;
     uint32_t *vs;
-    int rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, key_bits(skip), &ks, &vs);
+    int rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, key_bits(skip) + SIZE_BITS, &ks, &vs);
     if (rc) return rc;
     hipLaunchKernelGGL(k_bin_deposit<P>, dim3((unsigned)((n + BIN_CHUNK - 1) / BIN_CHUNK)), dim3(DEP_THREADS), 0, s,
                        a, ks, vs);
@@ -497,8 +507,9 @@ MHIP_DEV int wrapped_first(const Dep3Args &a, const Foot3 &fp, int c) {
 struct Vol3 {
     using Args = Dep3Args;
     static MHIP_DEV uint32_t skip(const Dep3Args &a) { return (uint32_t)a.nb * (uint32_t)a.nb * (uint32_t)a.nb; }
-    static MHIP_DEV uint32_t key(const Dep3Args &a, int64_t i) {
+    static MHIP_DEV uint32_t key(const Dep3Args &a, int64_t i, uint32_t &ncell) {
         Foot3 fp;
+        ncell = 0;
         if (!footprint3(a, i, fp)) return skip(a);
         int64_t cells = 1;
         bool fits = true;
@@ -511,6 +522,7 @@ struct Vol3 {
             a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;
             return skip(a);
         }
+        ncell = (uint32_t)cells;
         const uint32_t nb = (uint32_t)a.nb;
         return ((uint32_t)(wrapped_first(a, fp, 0) / BRICK3) * nb + (uint32_t)(wrapped_first(a, fp, 1) / BRICK3)) * nb +
                (uint32_t)(wrapped_first(a, fp, 2) / BRICK3);
