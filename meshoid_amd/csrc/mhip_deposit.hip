@@ -47,6 +47,10 @@ static_assert(BIN_CHUNK <= 65535, "run offsets and lengths are packed in 16 bits
 // brick's particles come largest footprint first and a wavefront's 64
 // footprints are alike (its lanes finish together)
 constexpr int SIZE_BITS = 11;
+// conservative footprints up to this many cells keep their weights in
+// registers between the total and the deposit (the window kernel is limited
+// to 3 waves per SIMD by its LDS, so the registers cost no occupancy)
+constexpr int WCACHE2 = 25, WCACHE3 = 27;
 constexpr uint32_t SIZE_MAX_RANK = (1u << SIZE_BITS) - 1;
 
 struct DepArgs {
@@ -227,9 +231,43 @@ struct Map2 {
     static MHIP_DEV void window(const DepArgs &a, uint32_t i, uint32_t key, double *win) {
         Foot fp;
         footprint(a, i, fp);
+        const int ox = (int)(key / (uint32_t)a.nb) * BRICK2, oy = (int)(key % (uint32_t)a.nb) * BRICK2;
+        const int wy = fp.gy1 - fp.gy0 + 1, ncell = (fp.gx1 - fp.gx0 + 1) * wy;
+        if (a.conservative && ncell <= WCACHE2) {
+            // small footprints: the weights stay in registers between the
+            // total (summed in foot_total's order) and the deposit
+            double wc[WCACHE2];
+            double tot = 0.0;
+            int gx = fp.gx0, gy = fp.gy0;
+#pragma unroll
+            for (int c = 0; c < WCACHE2; c++) {
+                wc[c] = 0.0;
+                if (c < ncell) {
+                    wc[c] = cell_weight(a, fp, gx, gy);
+                    tot += wc[c];
+                    if (++gy > fp.gy1) {
+                        gy = fp.gy0;
+                        gx++;
+                    }
+                }
+            }
+            if (tot == 0.0) return;
+            gx = fp.gx0;
+            gy = fp.gy0;
+#pragma unroll
+            for (int c = 0; c < WCACHE2; c++) {
+                if (c < ncell) {
+                    if (wc[c] != 0.0) atomicAdd(&win[(gx - ox) * WIN2 + (gy - oy)], wc[c] * fp.w / tot);
+                    if (++gy > fp.gy1) {
+                        gy = fp.gy0;
+                        gx++;
+                    }
+                }
+            }
+            return;
+        }
         const double tot = foot_total(a, fp);
         if (tot == 0.0) return;
-        const int ox = (int)(key / (uint32_t)a.nb) * BRICK2, oy = (int)(key % (uint32_t)a.nb) * BRICK2;
         for (int gx = fp.gx0; gx <= fp.gx1; gx++)
             for (int gy = fp.gy0; gy <= fp.gy1; gy++) {
                 const double k = cell_weight(a, fp, gx, gy);
@@ -542,13 +580,59 @@ struct Vol3 {
     static MHIP_DEV void window(const Dep3Args &a, uint32_t i, uint32_t key, double *win) {
         Foot3 fp;
         footprint3(a, i, fp);
-        const double tot = foot3_total(a, fp);
-        if (!(tot > 0.0)) return;
         // window index of the footprint's first cell on each axis (0..7)
         const uint32_t nb = (uint32_t)a.nb;
         const int b[3] = {(int)(key / (nb * nb)), (int)(key / nb % nb), (int)(key % nb)};
         int s[3];
         for (int c = 0; c < 3; c++) s[c] = wrapped_first(a, fp, c) - b[c] * BRICK3;
+        const int wy = fp.g1[1] - fp.g0[1] + 1, wz = fp.g1[2] - fp.g0[2] + 1;
+        const int ncell = (fp.g1[0] - fp.g0[0] + 1) * wy * wz;
+        if (ncell <= WCACHE3) {
+            // small footprints: the weights stay in registers between the
+            // total (summed in foot3_total's order) and the deposit
+            double wc[WCACHE3];
+            double tot = 0.0;
+            int gx = fp.g0[0], gy = fp.g0[1], gz = fp.g0[2];
+#pragma unroll
+            for (int c = 0; c < WCACHE3; c++) {
+                wc[c] = 0.0;
+                if (c < ncell) {
+                    wc[c] = weight3(a, fp, gx, gy, gz);
+                    tot += wc[c];
+                    if (++gz > fp.g1[2]) {
+                        gz = fp.g0[2];
+                        if (++gy > fp.g1[1]) {
+                            gy = fp.g0[1];
+                            gx++;
+                        }
+                    }
+                }
+            }
+            if (!(tot > 0.0)) return;
+            gx = fp.g0[0];
+            gy = fp.g0[1];
+            gz = fp.g0[2];
+#pragma unroll
+            for (int c = 0; c < WCACHE3; c++) {
+                if (c < ncell) {
+                    if (wc[c] != 0.0) {
+                        const int w = ((gx - fp.g0[0] + s[0]) * WIN3 + (gy - fp.g0[1] + s[1])) * WIN3 +
+                                      (gz - fp.g0[2] + s[2]);
+                        atomicAdd(&win[w], fp.f * wc[c] / tot);
+                    }
+                    if (++gz > fp.g1[2]) {
+                        gz = fp.g0[2];
+                        if (++gy > fp.g1[1]) {
+                            gy = fp.g0[1];
+                            gx++;
+                        }
+                    }
+                }
+            }
+            return;
+        }
+        const double tot = foot3_total(a, fp);
+        if (!(tot > 0.0)) return;
         for (int gx = fp.g0[0]; gx <= fp.g1[0]; gx++)
             for (int gy = fp.g0[1]; gy <= fp.g1[1]; gy++)
                 for (int gz = fp.g0[2]; gz <= fp.g1[2]; gz++) {
