@@ -154,6 +154,40 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_tiles(DepArgs a, int tw
                 a.big[atomicAdd(a.nbig, 1u)] = (uint32_t)i;
             continue;
         }
+        if (a.conservative && ntx == 1 && cells <= WCACHE2) {
+            // one tile holds the whole map (and footprint): small footprints
+            // keep their weights in registers between the total (summed in
+            // the order of the loop below) and the deposit
+            double wc[WCACHE2];
+            double tot = 0.0;
+            int gx = fp.gx0, gy = fp.gy0;
+#pragma unroll
+            for (int c = 0; c < WCACHE2; c++) {
+                wc[c] = 0.0;
+                if (c < cells) {
+                    wc[c] = cell_weight(a, fp, gx, gy);
+                    tot += wc[c];
+                    if (++gy > fp.gy1) {
+                        gy = fp.gy0;
+                        gx++;
+                    }
+                }
+            }
+            if (tot == 0.0) continue;
+            gx = fp.gx0;
+            gy = fp.gy0;
+#pragma unroll
+            for (int c = 0; c < WCACHE2; c++) {
+                if (c < cells) {
+                    if (wc[c] != 0.0) atomicAdd(&sg[(gx - tx0) * tw + (gy - ty0)], wc[c] * fp.w / tot);
+                    if (++gy > fp.gy1) {
+                        gy = fp.gy0;
+                        gx++;
+                    }
+                }
+            }
+            continue;
+        }
         const int gx0 = max(fp.gx0, tx0), gx1 = min(fp.gx1, tx1);
         const int gy0 = max(fp.gy0, ty0), gy1 = min(fp.gy1, ty1);
         if (gx0 > gx1 || gy0 > gy1) continue;
