@@ -212,7 +212,7 @@ __global__ __launch_bounds__(64) void k_rt_chunk(RtChunkArgs c) {
             const int64_t r = base + lane;
             bool hit = false;
             int64_t kk = 0;
-            double mh2 = 0.0, hinv = 0.0;
+            double mh2 = 0.0, hinv = 0.0, kap1 = 0.0, src1 = 0.0;
             RtFoot f;
             f.xs0 = f.xs1 = f.hs = 0.0;
             f.gx0 = f.gy0 = 0;
@@ -228,6 +228,12 @@ __global__ __launch_bounds__(64) void k_rt_chunk(RtChunkArgs c) {
                     hit = f.gx0 <= f.gx1 && f.gy0 <= f.gy1;
                     mh2 = m / (f.hs * f.hs);
                     hinv = 1.0 / f.hs;
+                    if (a.nf == 1) {
+                        // one band: kappa and the source term travel with
+                        // the particle instead of being read per update
+                        kap1 = a.skap[kk];
+                        src1 = a.ssrc[kk];
+                    }
                 }
             }
             uint64_t todo = __ballot(hit);
@@ -237,6 +243,7 @@ __global__ __launch_bounds__(64) void k_rt_chunk(RtChunkArgs c) {
                 const int64_t kp = __shfl(kk, src);
                 const double pxs0 = __shfl(f.xs0, src), pxs1 = __shfl(f.xs1, src), phs = __shfl(f.hs, src);
                 const double pmh2 = __shfl(mh2, src), phinv = __shfl(hinv, src);
+                const double pkap1 = __shfl(kap1, src), psrc1 = __shfl(src1, src);
                 const int pgx0 = __shfl(f.gx0, src), pgx1 = __shfl(f.gx1, src), pgy0 = __shfl(f.gy0, src),
                           pgy1 = __shfl(f.gy1, src);
                 const double hs2 = phs * phs;
@@ -252,11 +259,13 @@ __global__ __launch_bounds__(64) void k_rt_chunk(RtChunkArgs c) {
                     const double kern = rt_kernel2d(sqrt(rr) * phinv);
                     const int p = ((gx - tx0) * a.tw + (gy - ty0)) * a.nf;
                     for (int b = 0; b < a.nf; b++) {
-                        const double tau = a.skap[kp * a.nf + b] * kern * pmh2;
+                        const double kap = a.nf == 1 ? pkap1 : a.skap[kp * a.nf + b];
+                        const double sv = a.nf == 1 ? psrc1 : a.ssrc[kp * a.nf + b];
+                        const double tau = kap * kern * pmh2;
                         // exp(-tau) = 1 + expm1(-tau): one transcendental per update
                         const double em1 = expm1(-tau), fac1 = 1.0 + em1, fac2 = -em1;
                         T[p + b] += tau;
-                        I[p + b] = fac1 * I[p + b] + fac2 * a.ssrc[kp * a.nf + b];
+                        I[p + b] = fac1 * I[p + b] + fac2 * sv;
                     }
                 }
             }
