@@ -13,7 +13,7 @@ Prints ONE JSON line (the metric contract), with
                 measured with HIP events on the library's stream
   cpu_baseline  the reference algorithm on the host cores (scipy cKDTree --
                 the reference's own kNN call -- plus the FP64 oracle's HsmlIter
-                and gradient weights) on a 2^21-particle sample of the workload
+                and gradient weights) on the same 2^24 particles (--cpu-n)
   accuracy      h / rho / grad f errors against the FP64 oracle on a sample
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3]

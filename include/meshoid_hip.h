@@ -123,6 +123,14 @@ int mhip_knn_hsml(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int6
 int mhip_nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out);
 
 /*
+ * The same rows with float64 distances, recomputed from the positions the
+ * index was built from (the caller's buffer must still hold them): the
+ * float64 distances cKDTree.query returns (src/meshoid/Meshoid.py:157), rows
+ * ascending by (distance, index).  Works with or without MHIP_KNN_KEEP_DIST.
+ */
+int mhip_nlist_export_f64(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, double *dist_out);
+
+/*
  * Least-squares derivative operators over the list's rows.  Replaces
  * Meshoid.D (src/meshoid/Meshoid.py:237-268), Meshoid.D2 (:270-295) with
  * ComputeDWeights (:110-140) and gradient_weights
@@ -170,11 +178,13 @@ int mhip_kernel(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out);
 
 /*
  * Meshoid.get_kernel_weights (src/meshoid/Meshoid.py:172-177): W(d/h)
- * normalised per row, for a row-major (nq, k) float32 distance array (the
- * exported ngbdist) and float32 h[nq]; w float64[nq*k].  A row with a zero
- * kernel sum is NaN, as in the reference.
+ * normalised per row, for a row-major (nq, k) distance array (the exported
+ * ngbdist; float32, or float64 with dist_f64 = 1) and h[nq] (float32, or
+ * float64 with h_f64 = 1); w float64[nq*k].  A row with a zero kernel sum is
+ * NaN, as in the reference.
  */
-int mhip_kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w);
+int mhip_kernel_weights(mhip_ctx *ctx, const void *dist, int dist_f64, const void *h, int h_f64, int64_t nq, int k,
+                        double *w);
 
 /*
  * Meshoid.KernelAverage(f) (src/meshoid/Meshoid.py:369-381): per query row,
@@ -195,10 +205,12 @@ int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int
 
 /*
  * Meshoid.Integrate(f) (src/meshoid/Meshoid.py:333-350): sum_i vol_i f_i per
- * component, vol float32[n], f (n, K) float32/float64 (rows already masked);
- * out float64[K].  Deterministic two-level float64 reduction.
+ * component, vol[n] float32 (vol_f64 = 0) or float64, f (n, K)
+ * float32/float64 (rows already masked); out float64[K].  Deterministic
+ * two-level float64 reduction.
  */
-int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
+int mhip_integrate(mhip_ctx *ctx, const void *vol, int vol_f64, const void *f, int f_f64, int64_t n, int K,
+                   double *out);
 
 /*
  * GridSurfaceDensity (src/meshoid/grid_deposition.py:123-295), behind

@@ -30,7 +30,7 @@ EXPORTED = [
     "mhip_kernel_weights", "mhip_kernel_average", "mhip_kernel_variance", "mhip_integrate",
     "mhip_batch_create", "mhip_batch_run", "mhip_batch_info", "mhip_batch_destroy",
     "mhip_index_order", "mhip_nearest", "mhip_unique_inverse", "mhip_reconstruct", "mhip_nlist_subset",
-    "mhip_surface_density", "mhip_grid_density", "mhip_radtransfer",
+    "mhip_surface_density", "mhip_grid_density", "mhip_radtransfer", "mhip_nlist_export_f64",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -61,6 +61,7 @@ _SIGS = {
     "mhip_nlist_destroy": ([_p], _i),
     "mhip_knn_hsml": ([_p, _p, _p, _i64, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p], _i),
     "mhip_nlist_export": ([_p, _p, _p, _p], _i),
+    "mhip_nlist_export_f64": ([_p, _p, _p, _p], _i),
     "mhip_apply_D": ([_p, _p, _p, _i, _i64, _i, _i, _i, _p, _p, _p], _i),
     "mhip_gradient_weights": ([_p, _p, _i, _i64, _i, _p, _i64, _i, _p, _p, _d, _i, _i, _p, _p],
                               _i),
@@ -73,10 +74,10 @@ _SIGS = {
     "mhip_mlp_train_step": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _i, ctypes.c_float], _i),
     "mhip_mlp_pack": ([_p, _p, _p, _p, _p], _i),
     "mhip_mlp_blob_bytes": ([], _i64),
-    "mhip_kernel_weights": ([_p, _p, _p, _i64, _i, _p], _i),
+    "mhip_kernel_weights": ([_p, _p, _i, _p, _i, _i64, _i, _p], _i),
     "mhip_kernel_average": ([_p, _p, _p, _i, _i64, _p], _i),
     "mhip_kernel_variance": ([_p, _p, _p, _i, _i64, _i, _p], _i),
-    "mhip_integrate": ([_p, _p, _p, _i, _i64, _i, _p], _i),
+    "mhip_integrate": ([_p, _p, _i, _p, _i, _i64, _i, _p], _i),
     "mhip_batch_create": ([_p, _i, _p, _p, _i, _i, _pp], _i),
     "mhip_batch_run": ([_p, _p, _p, _p, _p, _p, _p, _p, _i], _i),
     "mhip_batch_info": ([_p, _p, _p, _p, _p, _p], _i),

@@ -81,6 +81,7 @@ int mhip_ctx_destroy(mhip_ctx *ctx) {
         (void)hipEventDestroy(r.stop);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
     delete ctx;
     return MHIP_OK;
 }
@@ -153,7 +154,19 @@ int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen) {
 
 int mhip_ctx_set_stream(mhip_ctx *ctx, void *stream) {
     if (!ctx) return MHIP_ERR_VALUE;
-    ctx->stream = (hipStream_t)stream;
+    hipStream_t ns = (hipStream_t)stream;
+    if (ns == ctx->stream) return MHIP_OK;
+    // the context's scratch and neighbour-list buffers may still be in use by
+    // work queued on the old stream: the new stream waits for it first
+    hipSetDevice(ctx->device);
+    if (!ctx->order_ev) {
+        int rc = check_hip(ctx, hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming), "event create");
+        if (rc) return rc;
+    }
+    int rc = check_hip(ctx, hipEventRecord(ctx->order_ev, ctx->stream), "event record");
+    if (!rc) rc = check_hip(ctx, hipStreamWaitEvent(ns, ctx->order_ev, 0), "stream wait");
+    if (rc) return rc;
+    ctx->stream = ns;
     return MHIP_OK;
 }
 
@@ -273,6 +286,13 @@ int mhip_nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, flo
     return nlist_export(ctx, nl, ngb_out, dist_out);
 }
 
+int mhip_nlist_export_f64(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, double *dist_out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(nl, "null neighbour list");
+    hipSetDevice(ctx->device);
+    return nlist_export64(ctx, nl, ngb_out, dist_out);
+}
+
 int mhip_apply_D(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows, int K,
                  int order, int weighted, float *out_d1, float *out_d2, int32_t *status) {
     if (!ctx) return MHIP_ERR_VALUE;
@@ -308,11 +328,12 @@ int mhip_kernel(mhip_ctx *ctx, const void *q, int f64, int64_t n, void *out) {
     return kernel_eval(ctx, q, f64 ? 1 : 0, n, out);
 }
 
-int mhip_kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w) {
+int mhip_kernel_weights(mhip_ctx *ctx, const void *dist, int dist_f64, const void *h, int h_f64, int64_t nq, int k,
+                        double *w) {
     if (!ctx) return MHIP_ERR_VALUE;
     MHIP_NEED(dist && h && w, "null argument");
     hipSetDevice(ctx->device);
-    return kernel_weights(ctx, dist, h, nq, k, w);
+    return kernel_weights(ctx, dist, dist_f64 ? 1 : 0, h, h_f64 ? 1 : 0, nq, k, w);
 }
 
 int mhip_kernel_average(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows,
@@ -333,11 +354,12 @@ int mhip_kernel_variance(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int
     return kernel_stat(ctx, nl, f, f_f64 ? 1 : 0, nf_rows, K, 1, out);
 }
 
-int mhip_integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out) {
+int mhip_integrate(mhip_ctx *ctx, const void *vol, int vol_f64, const void *f, int f_f64, int64_t n, int K,
+                   double *out) {
     if (!ctx) return MHIP_ERR_VALUE;
     MHIP_NEED(vol && f && out, "null argument");
     hipSetDevice(ctx->device);
-    return integrate(ctx, vol, f, f_f64 ? 1 : 0, n, K, out);
+    return integrate(ctx, vol, vol_f64 ? 1 : 0, f, f_f64 ? 1 : 0, n, K, out);
 }
 
 int mhip_nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int64_t nq, const double *m_rows,

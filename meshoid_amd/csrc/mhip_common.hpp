@@ -114,6 +114,7 @@ struct mhip_ctx {
     std::string last_error;
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    hipEvent_t order_ev = nullptr;  // orders work across a stream change
     // kernel timing: HIP events recorded on the context stream around each
     // launch region while prof is on (mhip_prof_enable / mhip_prof_read)
     int prof = 0;

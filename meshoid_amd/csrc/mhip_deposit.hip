@@ -452,7 +452,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit_big(DepArgs a) {
         for (int64_t c = threadIdx.x; c < cells; c += DEP_THREADS) {
             const int gx = fp.gx0 + (int)(c / wy), gy = fp.gy0 + (int)(c % wy);
             const double k = cell_weight(a, fp, gx, gy);
-            if (k != 0.0) atomicAdd(&a.grid[gx * a.res + gy], a.conservative ? k * fp.w / tot : k * fp.w);
+            if (k != 0.0) atomicAdd(&a.grid[(int64_t)gx * a.res + gy], a.conservative ? k * fp.w / tot : k * fp.w);
         }
     }
 }

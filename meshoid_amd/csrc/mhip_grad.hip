@@ -40,7 +40,48 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 
     const double qz = (double)a.sz[qp] + (a.p64 ? (double)a.lz[qp] : 0.0);
     const double box = (double)a.box;
     const float hh = a.h[s];
-    const double ih = hh > 0.f ? 1.0 / (double)hh : (double)INFINITY;
+    double hd = (double)hh;
+    if (a.refine_h && hh > 0.f) {
+        // The kNN pass solves h in float32 (relative error up to ~2.4e-7).
+        // Second-order fits are sensitive to h through the weights of the
+        // outermost neighbours (d ln W / d ln h = 3q / (1 - q) as q -> 1):
+        // a 2.4e-7 change of h moves their derivatives by up to ~6e-5 on
+        // well-conditioned rows.  One float64 Newton step on the exact pair
+        // distances takes h to ~1e-13 of the root of
+        // norm_d sum_j W(r_j / h) = k (kernel_density.py:36-56).
+        double sw = 0.0, sd = 0.0;
+        for (int j = gl; j < a.k; j += GW) {
+            const uint32_t p = a.nl_idx[(int64_t)j * a.nq + s];
+            double d[3] = {(double)a.sx[p] - qx, (double)a.sy[p] - qy, (double)a.sz[p] - qz};
+            if (a.p64) {
+                d[0] += (double)a.lx[p];
+                d[1] += (double)a.ly[p];
+                d[2] += (double)a.lz[p];
+            }
+            if (a.per) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) d[c] = wrap<double>(d[c], box);
+            }
+            const double q = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / hd;
+            double w, dw;
+            kernel_w_dw<double>(q, w, dw);
+            sw += w;
+            sd += q <= 1.0 ? dw * q : 0.0;
+        }
+        if (GW > 1) {
+#pragma unroll
+            for (int o = GW / 2; o > 0; o >>= 1) {
+                sw += __shfl_xor(sw, o);
+                sd += __shfl_xor(sd, o);
+            }
+        }
+        const double norm = hsml_norm(DIM);
+        const double fn = norm * sw - (double)a.k, dndh = -norm * sd / hd;
+        const double hn = hd - fn / dndh;
+        // a step beyond 1e-4 h means the float32 solve did not converge: keep it
+        if (dndh < 0.0 && fabs(hn - hd) <= 1e-4 * hd) hd = hn;
+    }
+    const double ih = hh > 0.f ? 1.0 / hd : (double)INFINITY;
     const FT *fs = (const FT *)a.fs;
     int32_t st = 0;
     for (int f0 = fb; f0 < fe; f0 += KC) {
@@ -318,6 +359,7 @@ void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, i
     a->out2 = nullptr;
     a->status = status;
     a->or_status = or_status;
+    a->refine_h = 0;
 }
 
 template <int DIM, int ORDER, typename FT>
@@ -388,6 +430,9 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     a.out2 = out2;
     a.status = status;
     a.or_status = 0;
+    // second-order fits take a float64-refined h (first-order ones are
+    // insensitive enough to the float32 h: <= 1e-6 relative)
+    a.refine_h = order == 2;
     prp.end();
     ProfRegion pra(ctx, order == 1 ? "apply_D1" : "apply_D2");
     const int gw = order == 1 ? apply_gw<1>() : apply_gw<2>();

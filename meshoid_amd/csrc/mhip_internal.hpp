@@ -57,6 +57,7 @@ int mlp_pack(mhip_ctx *ctx, const float *P, const float *mu, const float *sd, vo
 size_t mlp_blob_bytes();
 
 int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out);
+int nlist_export64(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, double *dist_out);
 int nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int64_t nq, const double *m_rows,
                  float *h_out, float *rho_out, float *vol_out, int32_t *status_out, mhip_nlist *nl);
 
@@ -107,6 +108,7 @@ struct ApplyArgs {
     float *out1, *out2;  // [row][K][DIM], [row][K][ND-DIM]
     int32_t *status;     // per row, may be null
     int or_status;       // 1: OR the bits into status instead of storing them
+    int refine_h;        // 1: one float64 Newton step on h before the weights
 };
 
 // every snapshot of a batch in one launch (table and block prefix on the
@@ -137,10 +139,11 @@ __global__ __launch_bounds__(256) void k_permute_fields_rows(const FT *__restric
 }
 
 // kernel-weighted neighbour statistics (mhip_kops.hip)
-int kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w);
+int kernel_weights(mhip_ctx *ctx, const void *dist, int dist_f64, const void *h, int h_f64, int64_t nq, int k,
+                   double *w);
 int kernel_stat(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int64_t nf_rows, int K,
                 int which, double *out);
-int integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out);
+int integrate(mhip_ctx *ctx, const void *vol, int vol_f64, const void *f, int f_f64, int64_t n, int K, double *out);
 
 // field rows (original order, K components each) -> sorted-space columns
 // [K][npad]: the j-th neighbours of a wave's 64 consecutive queries lie close

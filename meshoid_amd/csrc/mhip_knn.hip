@@ -1189,6 +1189,65 @@ int nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int
     return check_hip(ctx, hipGetLastError(), "subset launch");
 }
 
+// The same rows with float64 distances recomputed from the caller's
+// positions (original indices), the distance cKDTree.query returns
+// (src/meshoid/Meshoid.py:157): d = sqrt((dx^2 + dy^2) + dz^2) of the float64
+// offsets, minimum image when periodic; rows ascending by (distance, index).
+template <typename PT, bool PER>
+__global__ __launch_bounds__(256) void k_export64(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ qsorted,
+                                                 const uint32_t *__restrict__ qrow, const uint32_t *__restrict__ perm,
+                                                 const PT *__restrict__ pos, int dim, double box, int64_t nq, int k,
+                                                 int32_t *__restrict__ ngb_out, double *__restrict__ dist_out) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nq) return;
+    const int64_t row = qrow ? qrow[s] : perm[s];
+    const uint32_t qo = perm[qsorted ? qsorted[s] : (uint32_t)s];
+    auto d2_of = [&](uint32_t o) {
+        double r = 0.0;
+        for (int c = 0; c < dim; c++) {
+            double d = (double)pos[(int64_t)o * dim + c] - (double)pos[(int64_t)qo * dim + c];
+            if (PER) d = wrap<double>(d, box);
+            r = __dadd_rn(r, __dmul_rn(d, d));
+        }
+        return r;
+    };
+    for (int j = 0; j < k; j++) {
+        const uint32_t oj = perm[idx[(int64_t)j * nq + s]];
+        const double dj = d2_of(oj);
+        int rank = 0;
+        for (int l = 0; l < k; l++) {
+            if (l == j) continue;
+            const uint32_t ol = perm[idx[(int64_t)l * nq + s]];
+            const double dl = d2_of(ol);
+            rank += (dl < dj) || (dl == dj && ol < oj);
+        }
+        if (ngb_out) ngb_out[row * k + rank] = (int32_t)oj;
+        if (dist_out) dist_out[row * k + rank] = sqrt(dj);
+    }
+}
+
+int nlist_export64(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, double *dist_out) {
+    const mhip_index *ix = nl->index;
+    if (!ix || nl->nq < 1) return fail(ctx, -1, "neighbour list is empty");
+    if (!ix->pos_in) return fail(ctx, -1, "the index holds no positions");
+    ProfRegion pr(ctx, "nlist_export64");
+    const dim3 g((unsigned)((nl->nq + 255) / 256)), b(256);
+    const uint32_t *qs = nl->masked ? nl->qsorted : nullptr, *qr = nl->masked ? nl->qrow : nullptr;
+    const bool per = ix->box > 0;
+#define MHIP_EXP(PT, P)                                                                                        \
+    hipLaunchKernelGGL((k_export64<PT, P>), g, b, 0, ctx->stream, nl->idx, qs, qr, ix->perm, (const PT *)ix->pos_in, \
+                       ix->dim, ix->box, nl->nq, nl->k, ngb_out, dist_out)
+    if (ix->pos_f64) {
+        if (per) MHIP_EXP(double, true);
+        else MHIP_EXP(double, false);
+    } else {
+        if (per) MHIP_EXP(float, true);
+        else MHIP_EXP(float, false);
+    }
+#undef MHIP_EXP
+    return check_hip(ctx, hipGetLastError(), "export launch");
+}
+
 int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out) {
     const mhip_index *ix = nl->index;
     if (!ix || nl->nq < 1) return fail(ctx, -1, "neighbour list is empty");

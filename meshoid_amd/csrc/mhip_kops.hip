@@ -15,13 +15,13 @@
 namespace mhip {
 
 // W(r/h) / sum_j W(r_j/h) per row of a row-major (nq, k) distance array
-__global__ __launch_bounds__(256) void k_kernel_weights(const float *__restrict__ dist,
-                                                        const float *__restrict__ h, int64_t nq, int k,
-                                                        double *__restrict__ w) {
+template <typename DT, typename HT>
+__global__ __launch_bounds__(256) void k_kernel_weights(const DT *__restrict__ dist, const HT *__restrict__ h,
+                                                        int64_t nq, int k, double *__restrict__ w) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= nq) return;
     const double ih = 1.0 / (double)h[i];
-    const float *d = dist + i * k;
+    const DT *d = dist + i * k;
     double *o = w + i * k;
     double s = 0.0;
     for (int j = 0; j < k; j++) {
@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256) void k_kernel_variance(KopsArgs a) {
 // block folds the partials in a fixed order (deterministic)
 constexpr int INT_BLOCK = 256;
 
-template <typename FT>
-__global__ __launch_bounds__(INT_BLOCK) void k_integrate_partial(const float *__restrict__ vol,
+template <typename VT, typename FT>
+__global__ __launch_bounds__(INT_BLOCK) void k_integrate_partial(const VT *__restrict__ vol,
                                                                  const FT *__restrict__ f, int64_t n,
                                                                  int K, int c, double *__restrict__ part) {
     __shared__ double sh[INT_BLOCK];
@@ -142,11 +142,21 @@ __global__ __launch_bounds__(INT_BLOCK) void k_integrate_final(const double *__r
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-int kernel_weights(mhip_ctx *ctx, const float *dist, const float *h, int64_t nq, int k, double *w) {
+int kernel_weights(mhip_ctx *ctx, const void *dist, int dist_f64, const void *h, int h_f64, int64_t nq, int k,
+                   double *w) {
     if (nq < 0 || k < 1) return fail(ctx, -1, "bad kernel-weight shape");
     if (nq == 0) return 0;
-    hipLaunchKernelGGL(k_kernel_weights, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream,
-                       dist, h, nq, k, w);
+    const dim3 g((unsigned)((nq + 255) / 256)), b(256);
+#define MHIP_KW(DT, HT) \
+    hipLaunchKernelGGL((k_kernel_weights<DT, HT>), g, b, 0, ctx->stream, (const DT *)dist, (const HT *)h, nq, k, w)
+    if (dist_f64) {
+        if (h_f64) MHIP_KW(double, double);
+        else MHIP_KW(double, float);
+    } else {
+        if (h_f64) MHIP_KW(float, double);
+        else MHIP_KW(float, float);
+    }
+#undef MHIP_KW
     return check_hip(ctx, hipGetLastError(), "kernel weights launch");
 }
 
@@ -205,7 +215,7 @@ int kernel_stat(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, i
     return check_hip(ctx, hipGetLastError(), "kernel statistic launch");
 }
 
-int integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t n, int K, double *out) {
+int integrate(mhip_ctx *ctx, const void *vol, int vol_f64, const void *f, int f_f64, int64_t n, int K, double *out) {
     if (n < 0 || K < 1) return fail(ctx, -1, "bad integrate shape");
     const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + INT_BLOCK - 1) / INT_BLOCK));
     int rc = 0;
@@ -213,12 +223,17 @@ int integrate(mhip_ctx *ctx, const float *vol, const void *f, int f_f64, int64_t
     if (!part) return rc;
     ProfRegion pr(ctx, "integrate");
     for (int c = 0; c < K; c++) {
-        if (f_f64)
-            hipLaunchKernelGGL(k_integrate_partial<double>, dim3(nb), dim3(INT_BLOCK), 0, ctx->stream, vol,
-                               (const double *)f, n, K, c, part);
-        else
-            hipLaunchKernelGGL(k_integrate_partial<float>, dim3(nb), dim3(INT_BLOCK), 0, ctx->stream, vol,
-                               (const float *)f, n, K, c, part);
+#define MHIP_IP(VT, FT)                                                                                    \
+    hipLaunchKernelGGL((k_integrate_partial<VT, FT>), dim3(nb), dim3(INT_BLOCK), 0, ctx->stream, (const VT *)vol, \
+                       (const FT *)f, n, K, c, part)
+        if (vol_f64) {
+            if (f_f64) MHIP_IP(double, double);
+            else MHIP_IP(double, float);
+        } else {
+            if (f_f64) MHIP_IP(float, double);
+            else MHIP_IP(float, float);
+        }
+#undef MHIP_IP
         hipLaunchKernelGGL(k_integrate_final, dim3(1), dim3(INT_BLOCK), 0, ctx->stream, part, nb, out + c);
     }
     return check_hip(ctx, hipGetLastError(), "integrate launch");

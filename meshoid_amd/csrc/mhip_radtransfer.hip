@@ -262,8 +262,11 @@ __global__ __launch_bounds__(64) void k_rt_chunk(RtChunkArgs c) {
                         const double kap = a.nf == 1 ? pkap1 : a.skap[kp * a.nf + b];
                         const double sv = a.nf == 1 ? psrc1 : a.ssrc[kp * a.nf + b];
                         const double tau = kap * kern * pmh2;
-                        // exp(-tau) = 1 + expm1(-tau): one transcendental per update
-                        const double em1 = expm1(-tau), fac1 = 1.0 + em1, fac2 = -em1;
+                        // 1 - exp(-tau) = -expm1(-tau) stays accurate for small tau;
+                        // exp(-tau) itself comes from 1 + expm1(-tau) only while that
+                        // sum does not cancel (tau <= 1/2), from exp beyond
+                        const double em1 = expm1(-tau), fac2 = -em1;
+                        const double fac1 = tau <= 0.5 ? 1.0 + em1 : exp(-tau);
                         T[p + b] += tau;
                         I[p + b] = fac1 * I[p + b] + fac2 * sv;
                     }

@@ -126,13 +126,17 @@ class NeighborList:
         self.index = None
         self.keep_dist = False
 
-    def export(self, want_ngb=True, want_dist=True):
-        """(ngb int32 (nq,k), dist float32 (nq,k)) rows ascending, as cKDTree.query."""
+    def export(self, want_ngb=True, want_dist=True, f64=False):
+        """(ngb int32 (nq,k), dist (nq,k)) rows ascending, as cKDTree.query.
+        f64=True: float64 distances recomputed from the index's positions (the
+        distances cKDTree returns); otherwise the list's float32 distances."""
         dev = self.index.pos.device
         ngb = torch.empty((self.nq, self.k), dtype=torch.int32, device=dev) if want_ngb else None
-        dist = torch.empty((self.nq, self.k), dtype=torch.float32, device=dev) if want_dist else None
+        dt = torch.float64 if f64 else torch.float32
+        dist = torch.empty((self.nq, self.k), dtype=dt, device=dev) if want_dist else None
         ctx = context(dev)
-        raise_for(lib.mhip_nlist_export(ctx, self.h, ptr(ngb), ptr(dist)), ctx)
+        fn = lib.mhip_nlist_export_f64 if f64 else lib.mhip_nlist_export
+        raise_for(fn(ctx, self.h, ptr(ngb), ptr(dist)), ctx)
         return ngb, dist
 
     def __del__(self):
@@ -274,14 +278,15 @@ def kernel_dev(q):
 
 def kernel_weights_dev(dist: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
     """W(d/h) normalised per row (Meshoid.get_kernel_weights), float64 (nq, k)."""
-    _need(dist, (torch.float32,), "dist")
-    _need(h, (torch.float32,), "h")
+    _need(dist, (torch.float32, torch.float64), "dist")
+    _need(h, (torch.float32, torch.float64), "h")
     nq, k = dist.shape
     if h.shape[0] != nq:
         raise ValueError("h must have one entry per row of dist")
     w = torch.empty((nq, k), dtype=torch.float64, device=dist.device)
     ctx = context(dist.device)
-    raise_for(lib.mhip_kernel_weights(ctx, ptr(dist), ptr(h), nq, k, ptr(w)), ctx)
+    raise_for(lib.mhip_kernel_weights(ctx, ptr(dist), int(dist.dtype == torch.float64), ptr(h),
+                                      int(h.dtype == torch.float64), nq, k, ptr(w)), ctx)
     return w
 
 
@@ -313,7 +318,7 @@ def kernel_variance(nl: NeighborList, f: torch.Tensor) -> torch.Tensor:
 
 def integrate(vol: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
     """sum_i vol_i f_i per component (Meshoid.Integrate): float64 (K,)."""
-    _need(vol, (torch.float32,), "vol")
+    _need(vol, (torch.float32, torch.float64), "vol")
     _need(f, (torch.float32, torch.float64), "f")
     f2 = f.reshape(f.shape[0], -1).contiguous()
     if f2.shape[0] != vol.shape[0]:
@@ -321,8 +326,8 @@ def integrate(vol: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
     K = f2.shape[1]
     out = torch.empty(K, dtype=torch.float64, device=f.device)
     ctx = context(f.device)
-    raise_for(lib.mhip_integrate(ctx, ptr(vol), ptr(f2), int(f.dtype == torch.float64), f2.shape[0], K,
-                                 ptr(out)), ctx)
+    raise_for(lib.mhip_integrate(ctx, ptr(vol), int(vol.dtype == torch.float64), ptr(f2),
+                                 int(f.dtype == torch.float64), f2.shape[0], K, ptr(out)), ctx)
     return out
 
 

@@ -133,7 +133,7 @@ class Meshoid:
         self._np["ngbdist"] = v
 
     def _export(self):
-        ngb, dist = self._nl.export()
+        ngb, dist = self._nl.export(f64=True)
         self._dev["ngb"], self._dev["ngbdist"] = ngb, dist
         self._np["ngb"] = ngb.cpu().numpy().astype(np.int64)
         self._np["ngbdist"] = dist.cpu().numpy().astype(np.float64)
@@ -232,9 +232,9 @@ class Meshoid:
         h = self._dev.get("kernel_radius")
         dev = self._pos_d.device
         if dist is None:  # lists or radii assigned by the caller
-            dist = torch.from_numpy(np.ascontiguousarray(self.ngbdist, dtype=np.float32)).to(dev)
+            dist = torch.from_numpy(np.ascontiguousarray(self.ngbdist, dtype=np.float64)).to(dev)
         if h is None or "kernel_radius" in self._np:
-            h = torch.from_numpy(np.ascontiguousarray(self.kernel_radius, dtype=np.float32)).to(dev)
+            h = torch.from_numpy(np.ascontiguousarray(self.kernel_radius, dtype=np.float64)).to(dev)
         return engine.kernel_weights_dev(dist.contiguous(), h.contiguous()).cpu().numpy()
 
     def _need_device_list(self, what):
@@ -278,8 +278,8 @@ class Meshoid:
             self.vol = self.volnorm * self.kernel_radius**self.dim
         dev = self._pos_d.device
         vol = self._dev.get("vol") if "vol" not in self._np else None
-        if vol is None:
-            vol = torch.from_numpy(np.ascontiguousarray(self.vol, dtype=np.float32)).to(dev)
+        if vol is None:  # set by the caller or from kernel_radius: float64, as the reference's
+            vol = torch.from_numpy(np.ascontiguousarray(self.vol, dtype=np.float64)).to(dev)
         is_t = isinstance(f, torch.Tensor)
         tail = tuple(f.shape[1:])
         fd = as_device(f)

@@ -164,3 +164,36 @@ def test_binned_deposition_clustered(oracle):
         g = engine.grid_density(td(f), td(pos), td(h), c3, 10.0, 64, periodic, dx)
         go = oracle.grid_density(f, pos, np.maximum(h, dx), c3, 10.0, 64, 10.0 if periodic else -1.0)
         _close(g.cpu().numpy(), go, f"binned grid density periodic {periodic}")
+
+
+@pytest.mark.parametrize("cons", [False, True])
+def test_surface_density_map_beyond_int32_cells(oracle, cons):
+    """A 46500^2 map (more cells than int32 indexes) with footprints on the
+    block-per-particle path near the far corner: every deposit lands on its
+    own cells.  Each footprint is compared with the oracle's map of a 200^2
+    window at the same grid spacing (dx = 1)."""
+    torch = _torch()
+    from meshoid_amd import engine
+
+    R = 46500
+    S = float(R - 1)  # grid spacing 1
+    pts = np.array([[46400.3, 46380.7], [46450.6, 120.2], [150.4, 46460.9]])
+    h = np.array([40.0, 33.0, 37.5])  # footprints of ~5000 cells
+    f = np.array([1.0, 2.0, 0.5])
+    pos = np.concatenate([pts, np.zeros((3, 1))], axis=1)
+    say("surface density res", R, "conservative", cons)
+    g = engine.surface_density(torch.from_numpy(f).cuda(), torch.from_numpy(pos).cuda(),
+                               torch.from_numpy(h).cuda(), (S / 2, S / 2), S, R, 1.0, cons, hmin=0.0)
+    total = float(g.sum())
+    win = 0.0
+    for i in range(3):
+        g0 = np.floor(pts[i] - 100).astype(int)
+        sub = g[g0[0]:g0[0] + 200, g0[1]:g0[1] + 200].cpu().numpy()
+        win += sub.sum()
+        # the same footprint on a 200^2 map whose first point is cell g0
+        c_s = g0 + 99.5
+        go = oracle.grid_surface_density(f[i:i + 1], pos[i:i + 1], h[i:i + 1], c_s, 199.0, 200, cons)
+        _close(sub, go, f"footprint {i}")
+    del g
+    torch.cuda.empty_cache()
+    np.testing.assert_allclose(win, total, rtol=1e-12)

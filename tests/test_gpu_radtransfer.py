@@ -88,3 +88,26 @@ def test_radtransfer_no_particles_on_the_image():
     one = np.ones((50, 2))
     img = meshoid_amd.radtransfer(one, np.ones(50), one, x, np.full(50, 0.1), 16, 4.0, i0=i0)
     assert np.array_equal(img, i0)
+
+
+def test_radtransfer_thick_particles_before_bright_background(oracle):
+    """A bright background seen through particles of per-update optical depth
+    up to ~40 with little emission: exp(-tau) must not come from 1 + expm1(-tau)
+    there (it cancels to 0 beyond tau ~ 37).  Checked per pixel, relative to
+    the pixel itself."""
+    _torch()
+    import meshoid_amd
+
+    x = np.array([[0.1, -0.2, 1.0], [-0.3, 0.2, 0.0], [0.2, 0.1, -1.0]])
+    m = np.ones(3)
+    h = np.array([1.5, 1.2, 1.8])
+    kap = np.array([[22.0], [30.0], [12.0]])  # kernel2d(0) = 1.819: tau up to ~41 per particle
+    j = np.full((3, 1), 1e-3)
+    i0 = 1e40
+    img = meshoid_amd.radtransfer(j, m, kap, x, h, 32, 4.0, i0=i0)
+    want = oracle.radtransfer(j, m, kap, x, h, 32, 4.0, i0=i0)
+    pos = want > 0
+    err = np.abs(img[pos] - want[pos]) / want[pos]
+    say("per-pixel max rel", err.max(), "darkest pixel", want.min())
+    assert want.min() < 1e-10 * i0  # the sightlines are optically thick
+    assert err.max() <= 1e-11

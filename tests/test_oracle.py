@@ -151,16 +151,88 @@ def test_oracle_density_and_curl_div(oracle):
 
 
 def test_oracle_singular_moment_matrix(oracle):
-    # all neighbours on a plane -> E is singular in 3D
+    # all neighbours on a plane: every offset has dz = 0 exactly, so the z row
+    # and column of E are exactly zero and the elimination meets an exact zero
+    # pivot on every row (np.linalg.inv raises LinAlgError there)
     rng = np.random.default_rng(1)
     pos = rng.random((200, 3))
     pos[:, 2] = 0.5
     d, n = oracle.knn(pos, 32)
     h = oracle.hsml_iter(d)
-    _, st = oracle.gradient_weights(pos, n, h, np.arange(200), return_status=True)
-    # LU of an exactly rank-2 matrix in floating point: either an exact zero
-    # pivot (flagged) or a huge inverse, as with LAPACK
-    assert st.dtype == np.int32
+    G, st = oracle.gradient_weights(pos, n, h, np.arange(200), return_status=True)
+    assert (st == 1).all() and np.isnan(G).all()
+    # the same rows lifted off the plane are regular
+    pos[:, 2] = 0.5 + 0.01 * rng.standard_normal(200)
+    d, n = oracle.knn(pos, 32)
+    G, st = oracle.gradient_weights(pos, n, oracle.hsml_iter(d), np.arange(200), return_status=True)
+    assert (st == 0).all() and np.isfinite(G).all()
+
+
+def _two_shell_stencil(a, b, c=None):
+    """Query at the origin (row 0) with neighbours at +-a and +-b on every
+    axis and, optionally, at (+-c, +-c, 0) and its cyclic images."""
+    pts = [np.zeros(3)]
+    for r in (a, b):
+        for ax in range(3):
+            for sgn in (1.0, -1.0):
+                e = np.zeros(3)
+                e[ax] = sgn * r
+                pts.append(e)
+    if c is not None:
+        for ax in range(3):
+            for s1 in (1.0, -1.0):
+                for s2 in (1.0, -1.0):
+                    e = np.zeros(3)
+                    e[ax], e[(ax + 1) % 3] = s1 * c, s2 * c
+                    pts.append(e)
+    return np.array(pts)
+
+
+@pytest.mark.parametrize("h", [1.0, 0.9])
+def test_oracle_weighting_closed_form_order1(oracle, h):
+    """gradient_weights weights each pair by W(|dx|/h) (derivatives.py:50-51).
+    On the symmetric two-shell stencil E = diag(2 w_a a^2 + 2 w_b b^2) and
+    D(x^3)_x = (w_a a^4 + w_b b^4) / (w_a a^2 + w_b b^2): an unweighted fit,
+    W(r) or W(r/(2h)) each give a different value."""
+    a, b = 0.3, 0.8
+    pos = _two_shell_stencil(a, b)
+    k = len(pos)
+    ngb = np.arange(k)[None, :]
+    G = oracle.gradient_weights(pos, ngb, np.array([h]), np.array([0]))
+    f = pos[:, 0] ** 3
+    D = oracle.apply_weights(G, ngb, np.array([0]), f)[0]
+    wa, wb = oracle.kernel(a / h), oracle.kernel(b / h)
+    want = (wa * a**4 + wb * b**4) / (wa * a**2 + wb * b**2)
+    np.testing.assert_allclose(D, [want, 0.0, 0.0], rtol=1e-13, atol=1e-15)
+    # the alternatives a dropped or mis-scaled weight would give
+    for alt in (1.0, 1.0), (oracle.kernel(a / (2 * h)), oracle.kernel(b / (2 * h))):
+        wrong = (alt[0] * a**4 + alt[1] * b**4) / (alt[0] * a**2 + alt[1] * b**2)
+        assert abs(wrong - want) > 1e-3
+    Gu = oracle.gradient_weights(pos, ngb, np.array([h]), np.array([0]), weighted=False)
+    Du = oracle.apply_weights(Gu, ngb, np.array([0]), f)[0]
+    np.testing.assert_allclose(Du[0], (a**4 + b**4) / (a**2 + b**2), rtol=1e-13)
+
+
+def test_oracle_weighting_closed_form_order2(oracle):
+    """Order 2 (derivatives.py:83-101) on the two-shell stencil plus the 12
+    edge points (+-c, +-c, 0): the reflection symmetries split the 9x9 system
+    by parity, so the x row is (2 w_a a^4 + 2 w_b b^4 + 8 w_c c^4) /
+    (2 w_a a^2 + 2 w_b b^2 + 8 w_c c^2) with w_c = W(sqrt(2) c / h), and the
+    second-derivative rows of the odd field x^3 vanish."""
+    a, b, c, h = 0.3, 0.8, 0.45, 1.0
+    pos = _two_shell_stencil(a, b, c)
+    k = len(pos)
+    ngb = np.arange(k)[None, :]
+    G, st = oracle.gradient_weights(pos, ngb, np.array([h]), np.array([0]), order=2, return_status=True)
+    assert st[0] == 0
+    D = oracle.apply_weights(G, ngb, np.array([0]), pos[:, 0] ** 3)[0]
+    wa, wb, wc = oracle.kernel(a / h), oracle.kernel(b / h), oracle.kernel(np.sqrt(2) * c / h)
+    want = (2 * wa * a**4 + 2 * wb * b**4 + 8 * wc * c**4) / (2 * wa * a**2 + 2 * wb * b**2 + 8 * wc * c**2)
+    np.testing.assert_allclose(D[0], want, rtol=1e-12)
+    assert np.abs(D[1:]).max() < 1e-12
+    # and the fit is exact on the quadratic part of the basis
+    D2 = oracle.apply_weights(G, ngb, np.array([0]), pos[:, 0] * pos[:, 1])[0]
+    np.testing.assert_allclose(D2, [0, 0, 0, 0, 0, 0, 1, 0, 0], atol=1e-12)
 
 
 def test_oracle_curl_mask_quirk_raises(oracle):
