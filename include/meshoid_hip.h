@@ -131,6 +131,16 @@ int mhip_nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, flo
 int mhip_nlist_export_f64(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, double *dist_out);
 
 /*
+ * The smoothing lengths of the list's rows in float64, h_out double[nq] in
+ * row order: HsmlIter's root (src/meshoid/kernel_density.py:14-57, as
+ * TreeUpdate stores it at src/meshoid/Meshoid.py:165), refined with one
+ * float64 Newton step on the exact float64 distances of the final
+ * neighbours (h_out of mhip_knn_hsml is this value rounded to float32).
+ * The derivative and kernel-statistics entry points fit with it.
+ */
+int mhip_nlist_hsml(mhip_ctx *ctx, const mhip_nlist *nl, double *h_out);
+
+/*
  * Least-squares derivative operators over the list's rows.  Replaces
  * Meshoid.D (src/meshoid/Meshoid.py:237-268), Meshoid.D2 (:270-295) with
  * ComputeDWeights (:110-140) and gradient_weights

@@ -31,6 +31,7 @@ EXPORTED = [
     "mhip_batch_create", "mhip_batch_run", "mhip_batch_info", "mhip_batch_destroy",
     "mhip_index_order", "mhip_nearest", "mhip_unique_inverse", "mhip_reconstruct", "mhip_nlist_subset",
     "mhip_surface_density", "mhip_grid_density", "mhip_radtransfer", "mhip_nlist_export_f64",
+    "mhip_nlist_hsml",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -62,6 +63,7 @@ _SIGS = {
     "mhip_knn_hsml": ([_p, _p, _p, _i64, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p], _i),
     "mhip_nlist_export": ([_p, _p, _p, _p], _i),
     "mhip_nlist_export_f64": ([_p, _p, _p, _p], _i),
+    "mhip_nlist_hsml": ([_p, _p, _p], _i),
     "mhip_apply_D": ([_p, _p, _p, _i, _i64, _i, _i, _i, _p, _p, _p], _i),
     "mhip_gradient_weights": ([_p, _p, _i, _i64, _i, _p, _i64, _i, _p, _p, _d, _i, _i, _p, _p],
                               _i),

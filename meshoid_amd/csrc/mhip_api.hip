@@ -228,7 +228,7 @@ int mhip_nlist_create(mhip_nlist **out) {
 
 int mhip_nlist_destroy(mhip_nlist *nl) {
     if (!nl) return MHIP_OK;
-    void *arrs[] = {nl->qsorted, nl->qrow, nl->idx, nl->d2, nl->tau, nl->nlt, nl->h, nl->status, nl->amb};
+    void *arrs[] = {nl->qsorted, nl->qrow, nl->idx, nl->d2, nl->tau, nl->nlt, nl->h, nl->h64, nl->status, nl->amb};
     for (void *p : arrs)
         if (p) hipFree(p);
     delete nl;
@@ -284,6 +284,13 @@ int mhip_nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, flo
     MHIP_NEED(nl, "null neighbour list");
     hipSetDevice(ctx->device);
     return nlist_export(ctx, nl, ngb_out, dist_out);
+}
+
+int mhip_nlist_hsml(mhip_ctx *ctx, const mhip_nlist *nl, double *h_out) {
+    if (!ctx) return MHIP_ERR_VALUE;
+    MHIP_NEED(nl && h_out, "null argument");
+    hipSetDevice(ctx->device);
+    return nlist_hsml(ctx, nl, h_out);
 }
 
 int mhip_nlist_export_f64(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, double *dist_out) {

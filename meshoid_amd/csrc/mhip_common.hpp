@@ -210,7 +210,8 @@ struct mhip_nlist {
     float *d2 = nullptr;          // column-major [k][nq] squared distances
     float *tau = nullptr;         // k-th squared distance per slot
     int32_t *nlt = nullptr;       // count strictly below tau per slot
-    float *h = nullptr;           // smoothing length per slot
+    float *h = nullptr;           // smoothing length per slot (float32 copy of h64)
+    double *h64 = nullptr;        // smoothing length per slot, refined in float64
     int32_t *status = nullptr;    // per slot
     uint32_t *amb = nullptr;      // [0] count, [1..] slots flagged ambiguous (for the resolver)
     int masked = 0;               // qsorted/qrow valid (query rows are a subset/reordering)

@@ -23,10 +23,30 @@ namespace mhip {
 // ---------------------------------------------------------------------------
 
 
+// float64 offset of sorted particle p from the host particle (qx, qy, qz):
+// exact for float32 positions, minimum image when periodic
+MHIP_DEV void pair_offset(const ApplyArgs &a, uint32_t p, double qx, double qy, double qz, double (&d)[3]) {
+    d[0] = (double)a.sx[p] - qx;
+    d[1] = (double)a.sy[p] - qy;
+    d[2] = (double)a.sz[p] - qz;
+    if (a.p64) {
+        d[0] += (double)a.lx[p];
+        d[1] += (double)a.ly[p];
+        d[2] += (double)a.lz[p];
+    }
+    if (a.per) {
+        const double box = (double)a.box;
+#pragma unroll
+        for (int c = 0; c < 3; c++) d[c] = wrap<double>(d[c], box);
+    }
+}
+
 // one query slot s of the list, fields [fb, fe): KC fields per pass over the
 // neighbours.  With GW > 1 a group of GW consecutive lanes serves the query:
 // lane gl takes neighbours gl, gl + GW, ..., the moment matrix and right-hand
-// sides are summed over the group with shuffles, and lane 0 writes.
+// sides are summed over the group with shuffles, and lane 0 writes.  h is
+// the list's float64-refined smoothing length (the kNN pass refines it over
+// the final members, so every operator fits with the same h).
 template <int DIM, int ORDER, typename FT, int KC, int GW = 1>
 MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 0) {
     constexpr int ND = NDer<DIM, ORDER>::ND, NE = NDer<DIM, ORDER>::NE;
@@ -38,50 +58,8 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 
     const double qx = (double)a.sx[qp] + (a.p64 ? (double)a.lx[qp] : 0.0);
     const double qy = (double)a.sy[qp] + (a.p64 ? (double)a.ly[qp] : 0.0);
     const double qz = (double)a.sz[qp] + (a.p64 ? (double)a.lz[qp] : 0.0);
-    const double box = (double)a.box;
-    const float hh = a.h[s];
-    double hd = (double)hh;
-    if (a.refine_h && hh > 0.f) {
-        // The kNN pass solves h in float32 (relative error up to ~2.4e-7).
-        // Second-order fits are sensitive to h through the weights of the
-        // outermost neighbours (d ln W / d ln h = 3q / (1 - q) as q -> 1):
-        // a 2.4e-7 change of h moves their derivatives by up to ~6e-5 on
-        // well-conditioned rows.  One float64 Newton step on the exact pair
-        // distances takes h to ~1e-13 of the root of
-        // norm_d sum_j W(r_j / h) = k (kernel_density.py:36-56).
-        double sw = 0.0, sd = 0.0;
-        for (int j = gl; j < a.k; j += GW) {
-            const uint32_t p = a.nl_idx[(int64_t)j * a.nq + s];
-            double d[3] = {(double)a.sx[p] - qx, (double)a.sy[p] - qy, (double)a.sz[p] - qz};
-            if (a.p64) {
-                d[0] += (double)a.lx[p];
-                d[1] += (double)a.ly[p];
-                d[2] += (double)a.lz[p];
-            }
-            if (a.per) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) d[c] = wrap<double>(d[c], box);
-            }
-            const double q = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / hd;
-            double w, dw;
-            kernel_w_dw<double>(q, w, dw);
-            sw += w;
-            sd += q <= 1.0 ? dw * q : 0.0;
-        }
-        if (GW > 1) {
-#pragma unroll
-            for (int o = GW / 2; o > 0; o >>= 1) {
-                sw += __shfl_xor(sw, o);
-                sd += __shfl_xor(sd, o);
-            }
-        }
-        const double norm = hsml_norm(DIM);
-        const double fn = norm * sw - (double)a.k, dndh = -norm * sd / hd;
-        const double hn = hd - fn / dndh;
-        // a step beyond 1e-4 h means the float32 solve did not converge: keep it
-        if (dndh < 0.0 && fabs(hn - hd) <= 1e-4 * hd) hd = hn;
-    }
-    const double ih = hh > 0.f ? 1.0 / hd : (double)INFINITY;
+    const double hd = a.h[s];
+    const double ih = hd > 0.0 ? 1.0 / hd : (double)INFINITY;
     const FT *fs = (const FT *)a.fs;
     int32_t st = 0;
     for (int f0 = fb; f0 < fe; f0 += KC) {
@@ -96,18 +74,7 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 
             for (int i = 0; i < ND; i++) B[f][i] = 0.0;
             fc[f] = f0 + f < a.K ? fs[(int64_t)(f0 + f) * a.npad + qp] : FT(0);
         }
-        for (int j = gl; j < a.k; j += GW) {
-            const uint32_t p = a.nl_idx[(int64_t)j * a.nq + s];
-            double d[3] = {(double)a.sx[p] - qx, (double)a.sy[p] - qy, (double)a.sz[p] - qz};
-            if (a.p64) {
-                d[0] += (double)a.lx[p];
-                d[1] += (double)a.ly[p];
-                d[2] += (double)a.lz[p];
-            }
-            if (a.per) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) d[c] = wrap<double>(d[c], box);
-            }
+        auto pair = [&](uint32_t p, const double (&d)[3]) {
             double w = 1.0;
             if (a.weighted) w = kernel_w<double>(sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
             double av[ND];
@@ -128,6 +95,12 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 
                     for (int i = 0; i < ND; i++) B[f][i] = fma(wa[i], df, B[f][i]);
                 }
             }
+        };
+        for (int j = gl; j < a.k; j += GW) {
+            const uint32_t p = a.nl_idx[(int64_t)j * a.nq + s];
+            double d[3];
+            pair_offset(a, p, qx, qy, qz, d);
+            pair(p, d);
         }
         if (GW > 1) {
 #pragma unroll
@@ -244,8 +217,8 @@ __global__ __launch_bounds__(256) void k_apply_wide(const ApplyArgs *__restrict_
     const uint32_t qp = a.qsorted ? a.qsorted[sc] : (uint32_t)sc;
     const uint32_t row = a.qrow ? a.qrow[sc] : a.perm[sc];
     const double qx = (double)a.sx[qp], qy = (double)a.sy[qp], qz = (double)a.sz[qp];
-    const float hh = a.h[sc];
-    const double ih = hh > 0.f ? 1.0 / (double)hh : (double)INFINITY;
+    const double hd = a.h[sc];
+    const double ih = hd > 0.0 ? 1.0 / hd : (double)INFINITY;
     const int k = a.k;
     // phase 1: w_j a_j and the moment matrix
     double E[6] = {0, 0, 0, 0, 0, 0};
@@ -348,7 +321,7 @@ void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, i
     a->qsorted = nl->masked ? nl->qsorted : nullptr;
     a->qrow = nl->masked ? nl->qrow : nullptr;
     a->perm = ix->perm;
-    a->h = nl->h;
+    a->h = nl->h64;
     a->nq = nl->nq;
     a->npad = ix->npad;
     a->k = nl->k;
@@ -359,7 +332,6 @@ void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, i
     a->out2 = nullptr;
     a->status = status;
     a->or_status = or_status;
-    a->refine_h = 0;
 }
 
 template <int DIM, int ORDER, typename FT>
@@ -419,7 +391,7 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     a.qsorted = nl->masked ? nl->qsorted : nullptr;
     a.qrow = nl->masked ? nl->qrow : nullptr;
     a.perm = ix->perm;
-    a.h = nl->h;
+    a.h = nl->h64;
     a.nq = nl->nq;
     a.npad = ix->npad;
     a.k = nl->k;
@@ -430,9 +402,6 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     a.out2 = out2;
     a.status = status;
     a.or_status = 0;
-    // second-order fits take a float64-refined h (first-order ones are
-    // insensitive enough to the float32 h: <= 1e-6 relative)
-    a.refine_h = order == 2;
     prp.end();
     ProfRegion pra(ctx, order == 1 ? "apply_D1" : "apply_D2");
     const int gw = order == 1 ? apply_gw<1>() : apply_gw<2>();

@@ -58,6 +58,7 @@ size_t mlp_blob_bytes();
 
 int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out);
 int nlist_export64(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, double *dist_out);
+int nlist_hsml(mhip_ctx *ctx, const mhip_nlist *nl, double *h_out);
 int nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int64_t nq, const double *m_rows,
                  float *h_out, float *rho_out, float *vol_out, int32_t *status_out, mhip_nlist *nl);
 
@@ -99,7 +100,7 @@ struct ApplyArgs {
     int per, p64;
     const uint32_t *nl_idx;
     const uint32_t *qsorted, *qrow, *perm;
-    const float *h;
+    const double *h;     // smoothing length per slot (float64-refined)
     int64_t nq, npad;
     int k;
     const void *fs;  // [K][npad] sorted-space fields
@@ -108,7 +109,6 @@ struct ApplyArgs {
     float *out1, *out2;  // [row][K][DIM], [row][K][ND-DIM]
     int32_t *status;     // per row, may be null
     int or_status;       // 1: OR the bits into status instead of storing them
-    int refine_h;        // 1: one float64 Newton step on h before the weights
 };
 
 // every snapshot of a batch in one launch (table and block prefix on the

@@ -206,6 +206,7 @@ struct KnnArgs {
     float *tau;
     int32_t *nlt;
     float *h_slot;
+    double *h64_slot;
     int32_t *st_slot;
     uint32_t *nl_idx;
     float *nl_d2;
@@ -249,6 +250,61 @@ MHIP_DEV void pad_list(const KnnArgs &a, int cnt, uint32_t qp, int64_t slot) {
         a.nl_idx[(int64_t)j * a.nq + slot] = qp;
         if (a.nl_d2) a.nl_d2[(int64_t)j * a.nq + slot] = INFINITY;
     }
+}
+
+// One float64 Newton step on the smoothing length over the final member
+// list, member(j) giving the sorted index of member j < k.  The float32
+// solve below is at the float32 noise floor (relative error up to ~2.4e-7,
+// from the float32 distances); the derivative fits are sensitive to h through
+// the weights of the outermost neighbours (d ln W / d ln h = 3q / (1 - q) as
+// q -> 1), so a 2.4e-7 error of h moves second-order derivatives by up to
+// ~6e-5, and first-order derivatives of rough fields by ~4e-5, on
+// well-conditioned rows.  One step on the exact float64 pair distances
+// (float32 positions are exact in float64; float64 inputs are hi + lo) takes
+// h to ~1e-13 of the root of norm_d sum_j W(r_j / h) = k
+// (kernel_density.py:36-56).  A step beyond 1e-4 h means the float32 solve
+// did not converge; h is then kept.
+template <bool PER, bool P64, typename F>
+MHIP_DEV double refine_h64(const KnnArgs &a, const QPos &q, double hd, F member) {
+    const double qx = (double)q.x + (double)q.lx, qy = (double)q.y + (double)q.ly, qz = (double)q.z + (double)q.lz;
+    const double box = (double)a.t.box, ih = 1.0 / hd;
+    double sw = 0.0, sd = 0.0;
+    for (int j = 0; j < a.k; j++) {
+        const uint32_t p = member(j);
+        double d[3] = {(double)a.t.sx[p] - qx, (double)a.t.sy[p] - qy, (double)a.t.sz[p] - qz};
+        if (P64) {
+            d[0] += (double)a.t.lx[p];
+            d[1] += (double)a.t.ly[p];
+            d[2] += (double)a.t.lz[p];
+        }
+        if (PER) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) d[c] = wrap<double>(d[c], box);
+        }
+        const double qq = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih;
+        double w, dw;
+        kernel_w_dw<double>(qq, w, dw);
+        sw += w;
+        sd += qq <= 1.0 ? dw * qq : 0.0;
+    }
+    const double norm = hsml_norm(a.dim);
+    // n(h) grows with h: dn/dh = -norm sum_j W'(q_j) q_j / h > 0
+    const double fn = norm * sw - (double)a.k, dndh = -norm * sd * ih;
+    const double hn = hd - fn / dndh;
+    return (dndh > 0.0 && fabs(hn - hd) <= 1e-4 * hd) ? hn : hd;
+}
+
+// h, density and volume of one output row (Meshoid.py:165-170)
+MHIP_DEV void write_hsml(const KnnArgs &a, int64_t slot, uint32_t row, double h) {
+    a.h_slot[slot] = (float)h;
+    a.h64_slot[slot] = h;
+    const double vn = a.dim == 3 ? 4.18879020478639098 : (a.dim == 2 ? 3.14159265358979324 : 2.0);
+    const double m = a.m_rows ? a.m_rows[row] : 1.0;
+    const double hp = a.dim == 3 ? h * h * h : (a.dim == 2 ? h * h : h);
+    const double rho = (double)a.k * m / (vn * hp);
+    if (a.h_out) a.h_out[row] = (float)h;
+    if (a.rho_out) a.rho_out[row] = (float)rho;
+    if (a.vol_out) a.vol_out[row] = (float)(m / rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -635,15 +691,8 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     if (live) {
         a.tau[slot] = tau;
         a.nlt[slot] = nlt;
-        a.h_slot[slot] = h;
-        const double vn = a.dim == 3 ? 4.18879020478639098 : (a.dim == 2 ? 3.14159265358979324 : 2.0);
-        const double m = a.m_rows ? a.m_rows[row] : 1.0;
-        const double hd = (double)h, hp = a.dim == 3 ? hd * hd * hd : (a.dim == 2 ? hd * hd : hd);
-        const double rho = (double)a.k * m / (vn * hp);
-        if (a.h_out) a.h_out[row] = h;
-        if (a.rho_out) a.rho_out[row] = (float)rho;
-        if (a.vol_out) a.vol_out[row] = (float)(m / rho);
     }
+    double h64 = (double)h;
     // members: a candidate at or below tau passed its lane's queue filter
     // when its leaf was scanned, so it lies in a recorded leaf; leaves no
     // final ball reaches are skipped
@@ -706,10 +755,22 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
             else if (a.nl_d2)
                 for (int j = cnt; j < a.k; j++) a.nl_d2[(int64_t)j * a.nq + slot] = INFINITY;
         }
+        // float64 refinement over the members just collected; ambiguous rows
+        // (whose capped collection may hold the wrong ties) are refined by
+        // the resolver once it has rebuilt them
+        if (live && cnt == a.k && h > 0.f && !(st & (ST_HSML_NOCONV | ST_AMBIGUOUS))) {
+            if (STAGE)
+                h64 = refine_h64<PER, P64>(a, q, h64, [&](int j) { return s_buf[w][j][lane]; });
+            else
+                h64 = refine_h64<PER, P64>(a, q, h64, [&](int j) { return a.nl_idx[(int64_t)j * a.nq + slot]; });
+        }
     } else {
         st |= ST_REDO;
     }
-    if (live) a.st_slot[slot] = st;
+    if (live) {
+        write_hsml(a, slot, row, h64);
+        a.st_slot[slot] = st;
+    }
     // ambiguous queries for the resolver, one vector atomic per wavefront by
     // its lowest active lane (lanes that missed a radius seed left earlier;
     // in the redo launch only the recomputed lanes are new)
@@ -754,6 +815,11 @@ __global__ __launch_bounds__(256) void k_knn_pass2(KnnArgs a) {
     if (cnt != a.k) {
         st |= ST_COUNT;
         pad_list(a, cnt, qp, slot);
+    } else if (a.h_slot[slot] > 0.f && !(st & ST_HSML_NOCONV)) {
+        const uint32_t row = a.qrow ? a.qrow[slot] : a.perm[slot];
+        const double h64 = refine_h64<PER, P64>(a, q, (double)a.h_slot[slot],
+                                                [&](int j) { return a.nl_idx[(int64_t)j * a.nq + slot]; });
+        write_hsml(a, slot, row, h64);
     }
     a.st_slot[slot] = st;
 }
@@ -849,6 +915,12 @@ MHIP_DEV void resolve_slot(const KnnArgs &a, const PT *__restrict__ pos_in, int6
         pad_list(a, min(cnt, a.k), qp, slot);
     } else {
         st &= ~ST_COUNT;
+        if (a.h_slot[slot] > 0.f && !(st & ST_HSML_NOCONV)) {
+            const uint32_t row = a.qrow ? a.qrow[slot] : a.perm[slot];
+            const double h64 = refine_h64<PER, P64>(a, q, (double)a.h_slot[slot],
+                                                    [&](int j) { return a.nl_idx[(int64_t)j * nq + slot]; });
+            write_hsml(a, slot, row, h64);
+        }
     }
     a.st_slot[slot] = st;
 }
@@ -925,6 +997,7 @@ static int nl_reserve(mhip_ctx *ctx, mhip_nlist *nl, int64_t nq, int k) {
         for (auto p : arr)
             if ((rc = nl_grow(ctx, p, nq * 4))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->amb, (nq + 1) * 4))) return rc;
+        if ((rc = nl_grow(ctx, (void **)&nl->h64, nq * 8))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->idx, (size_t)nq * k * 4))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->d2, (size_t)nq * k * 4))) return rc;
         nl->cap = nq;
@@ -1015,6 +1088,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     a.tau = nl->tau;
     a.nlt = nl->nlt;
     a.h_slot = nl->h;
+    a.h64_slot = nl->h64;
     a.st_slot = nl->status;
     a.nl_idx = nl->idx;
     a.nl_d2 = nl->has_d2 ? nl->d2 : nullptr;
@@ -1136,12 +1210,12 @@ __global__ __launch_bounds__(256) void k_export(const uint32_t *__restrict__ idx
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_nl_subset(
     const uint32_t *__restrict__ fidx, const float *__restrict__ fd2, const float *__restrict__ ftau,
-    const int32_t *__restrict__ fnlt, const float *__restrict__ fh, const int32_t *__restrict__ fst,
-    int64_t nfull, const uint32_t *__restrict__ qsorted, const uint32_t *__restrict__ qrow, int64_t nq, int k,
-    int dim, const double *__restrict__ m_rows, uint32_t *__restrict__ idx, float *__restrict__ d2,
-    float *__restrict__ tau, int32_t *__restrict__ nlt, float *__restrict__ hs, int32_t *__restrict__ st,
-    float *__restrict__ h_out, float *__restrict__ rho_out, float *__restrict__ vol_out,
-    int32_t *__restrict__ status_out) {
+    const int32_t *__restrict__ fnlt, const float *__restrict__ fh, const double *__restrict__ fh64,
+    const int32_t *__restrict__ fst, int64_t nfull, const uint32_t *__restrict__ qsorted,
+    const uint32_t *__restrict__ qrow, int64_t nq, int k, int dim, const double *__restrict__ m_rows,
+    uint32_t *__restrict__ idx, float *__restrict__ d2, float *__restrict__ tau, int32_t *__restrict__ nlt,
+    float *__restrict__ hs, double *__restrict__ hs64, int32_t *__restrict__ st, float *__restrict__ h_out,
+    float *__restrict__ rho_out, float *__restrict__ vol_out, int32_t *__restrict__ status_out) {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= nq) return;
     const uint32_t f = qsorted[s], row = qrow[s];
@@ -1149,17 +1223,18 @@ __global__ __launch_bounds__(256) void k_nl_subset(
         idx[(int64_t)j * nq + s] = fidx[(int64_t)j * nfull + f];
         if (d2) d2[(int64_t)j * nq + s] = fd2[(int64_t)j * nfull + f];
     }
-    const float h = fh[f];
+    const double h = fh64[f];
     tau[s] = ftau[f];
     nlt[s] = fnlt[f];
-    hs[s] = h;
+    hs[s] = fh[f];
+    hs64[s] = h;
     st[s] = fst[f];
     // the main kernel's density and volume arithmetic
     const double vn = dim == 3 ? 4.18879020478639098 : (dim == 2 ? 3.14159265358979324 : 2.0);
     const double m = m_rows ? m_rows[row] : 1.0;
-    const double hd = (double)h, hp = dim == 3 ? hd * hd * hd : (dim == 2 ? hd * hd : hd);
+    const double hp = dim == 3 ? h * h * h : (dim == 2 ? h * h : h);
     const double rho = (double)k * m / (vn * hp);
-    if (h_out) h_out[row] = h;
+    if (h_out) h_out[row] = (float)h;
     if (rho_out) rho_out[row] = (float)rho;
     if (vol_out) vol_out[row] = (float)(m / rho);
     if (status_out) status_out[row] = fst[f];
@@ -1183,9 +1258,10 @@ int nlist_subset(mhip_ctx *ctx, const mhip_nlist *full, const int64_t *qidx, int
     if ((rc = query_setup(ctx, ix, qidx, nq, nl))) return rc;
     ProfRegion pr(ctx, "nlist_subset");
     hipLaunchKernelGGL(k_nl_subset, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream, full->idx,
-                       full->has_d2 ? full->d2 : nullptr, full->tau, full->nlt, full->h, full->status, full->nq,
-                       nl->qsorted, nl->qrow, nq, k, ix->dim, m_rows, nl->idx, full->has_d2 ? nl->d2 : nullptr,
-                       nl->tau, nl->nlt, nl->h, nl->status, h_out, rho_out, vol_out, status_out);
+                       full->has_d2 ? full->d2 : nullptr, full->tau, full->nlt, full->h, full->h64, full->status,
+                       full->nq, nl->qsorted, nl->qrow, nq, k, ix->dim, m_rows, nl->idx,
+                       full->has_d2 ? nl->d2 : nullptr, nl->tau, nl->nlt, nl->h, nl->h64, nl->status, h_out, rho_out,
+                       vol_out, status_out);
     return check_hip(ctx, hipGetLastError(), "subset launch");
 }
 
@@ -1246,6 +1322,22 @@ int nlist_export64(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, double
     }
 #undef MHIP_EXP
     return check_hip(ctx, hipGetLastError(), "export launch");
+}
+
+// the list's float64 smoothing lengths in output-row order
+__global__ __launch_bounds__(256) void k_rows_h64(const double *__restrict__ h64, const uint32_t *__restrict__ qrow,
+                                                 const uint32_t *__restrict__ perm, int64_t nq,
+                                                 double *__restrict__ out) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nq) return;
+    out[qrow ? qrow[s] : perm[s]] = h64[s];
+}
+
+int nlist_hsml(mhip_ctx *ctx, const mhip_nlist *nl, double *h_out) {
+    if (!nl->index || nl->nq < 1) return fail(ctx, -1, "neighbour list is empty");
+    hipLaunchKernelGGL(k_rows_h64, dim3((unsigned)((nl->nq + 255) / 256)), dim3(256), 0, ctx->stream, nl->h64,
+                       nl->masked ? nl->qrow : nullptr, nl->index->perm, nl->nq, h_out);
+    return check_hip(ctx, hipGetLastError(), "hsml copy launch");
 }
 
 int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *dist_out) {

@@ -39,7 +39,7 @@ struct KopsArgs {
     double box;
     int per, p64;
     const uint32_t *nl_idx, *qsorted, *qrow, *perm;
-    const float *h;
+    const double *h;  // float64-refined smoothing length per slot
     int64_t nq;
     int k;
     const void *fs;  // [K][npad] sorted-space field
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_kernel_average(KopsArgs a) {
     const uint32_t row = a.qrow ? a.qrow[s] : a.perm[s];
     const double qx = pos_d(a.sx, a.lx, a.p64, qp), qy = pos_d(a.sy, a.ly, a.p64, qp),
                  qz = pos_d(a.sz, a.lz, a.p64, qp);
-    const double ih = 1.0 / (double)a.h[s];
+    const double ih = 1.0 / a.h[s];
     double sw = 0.0, swf = 0.0;
     for (int j = 0; j < a.k; j++) {
         const uint32_t p = a.nl_idx[(int64_t)j * a.nq + s];
@@ -192,7 +192,7 @@ int kernel_stat(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, i
     a.qsorted = nl->masked ? nl->qsorted : nullptr;
     a.qrow = nl->masked ? nl->qrow : nullptr;
     a.perm = ix->perm;
-    a.h = nl->h;
+    a.h = nl->h64;
     a.nq = nl->nq;
     a.k = nl->k;
     a.fs = fs;

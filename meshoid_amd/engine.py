@@ -139,6 +139,14 @@ class NeighborList:
         raise_for(fn(ctx, self.h, ptr(ngb), ptr(dist)), ctx)
         return ngb, dist
 
+    def h64(self):
+        """float64 smoothing lengths of the rows (nq,), refined over the final
+        neighbours; the derivative operators fit with these."""
+        out = torch.empty(self.nq, dtype=torch.float64, device=self.index.pos.device)
+        ctx = context(out.device)
+        raise_for(lib.mhip_nlist_hsml(ctx, self.h, ptr(out)), ctx)
+        return out
+
     def __del__(self):
         if getattr(self, "h", None):
             lib.mhip_nlist_destroy(self.h)

@@ -207,7 +207,8 @@ class Meshoid:
         else:
             if qidx is not None and self._nl is full:
                 self._nl = None  # keep the unmasked list intact
-            out = engine.knn_hsml(self.tree, self.des_ngb, qidx=qidx, m_rows=m_d, keep_dist=True,
+            # ngbdist is exported in float64 from the positions: the list keeps no distances
+            out = engine.knn_hsml(self.tree, self.des_ngb, qidx=qidx, m_rows=m_d, keep_dist=False,
                                   nlist=self._nl)
             if qidx is None:
                 self._nl_full = out["nlist"]
@@ -218,8 +219,11 @@ class Meshoid:
         self._nl = out["nlist"]
         for key in ("ngb", "ngbdist", "kernel_radius", "density", "vol"):
             self._np.pop(key, None)
-        self._dev = {"kernel_radius": out["h"], "density": out["rho"], "vol": out["vol"],
-                     "status": out["status"]}
+        # float64 h of the final neighbours (HsmlIter's root to ~1e-13), and
+        # density and volume from it in float64, as Meshoid.py:165-170
+        h = self._nl.h64()
+        rho = self.des_ngb * m_d / (self.volnorm * h**self.dim)
+        self._dev = {"kernel_radius": h, "density": rho, "vol": m_d / rho, "status": out["status"]}
 
     def get_kernel_weights(self):
         """W(r_ij / h_i) normalised over each row's neighbours

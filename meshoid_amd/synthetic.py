@@ -74,7 +74,7 @@ def grf_grid(ngrid=256, box=100.0, slope=-3.0, seed=0):
     amp = np.zeros_like(k2)
     nz = k2 > 0
     amp[nz] = k2[nz] ** (slope / 4.0)  # sqrt(P(k)) with P ~ |k|^slope
-    field = np.fft.irfftn(wk * amp, s=white.shape)
+    field = np.fft.irfftn(wk * amp, s=white.shape, axes=(0, 1, 2))
     field -= field.mean()
     return field / field.std()
 

@@ -177,7 +177,7 @@ def test_surface_density_map_beyond_int32_cells(oracle, cons):
 
     R = 46500
     S = float(R - 1)  # grid spacing 1
-    pts = np.array([[46400.3, 46380.7], [46450.6, 120.2], [150.4, 46460.9]])
+    pts = np.array([[46380.3, 46370.7], [46390.6, 120.2], [150.4, 46385.9]])
     h = np.array([40.0, 33.0, 37.5])  # footprints of ~5000 cells
     f = np.array([1.0, 2.0, 0.5])
     pos = np.concatenate([pts, np.zeros((3, 1))], axis=1)

@@ -184,10 +184,18 @@ def test_knn_large_clustered_invariants():
     assert bool((ngb[:, 0].long() == torch.arange(1 << 22, device="cuda")).all() |
                 (dist[:, 0] == 0).all())
     assert bool((dist[:, 1:] >= dist[:, :-1]).all())
-    # the kernel sum at h equals des_ngb (float64 recomputation on device)
-    h = out["h"].double()
-    q = dist.double() / h[:, None]
-    W = torch.where(q <= 0.5, 1 - 6 * q**2 + 6 * q**3, torch.where(q <= 1, 2 * (1 - q) ** 3, 0 * q))
-    nn = 32 / 3 * W.sum(1)
-    assert float((nn - 32).abs().max()) < 1e-3
+    # the kernel sum at h equals des_ngb: over the float32 distances and h to
+    # float32 rounding, and over the float64 distances with the list's
+    # float64-refined h to ~1e-12
+    def nsum(d, h):
+        q = d / h[:, None]
+        W = torch.where(q <= 0.5, 1 - 6 * q**2 + 6 * q**3, torch.where(q <= 1, 2 * (1 - q) ** 3, 0 * q))
+        return 32 / 3 * W.sum(1)
+
+    assert float((nsum(dist.double(), out["h"].double()) - 32).abs().max()) < 1e-3
+    _, d64 = out["nlist"].export(want_ngb=False, f64=True)
+    h64 = out["nlist"].h64()
+    err = float((nsum(d64, h64) - 32).abs().max())
+    say("max |n(h) - 32| with float64 distances and h:", err)
+    assert err < 1e-9
     say("ambiguous rows resolved in fp64:", int(((st & 1) != 0).sum()))
