@@ -223,6 +223,21 @@ int mhip_integrate(mhip_ctx *ctx, const void *vol, int vol_f64, const void *f, i
                    double *out);
 
 /*
+ * mhip_integrate without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, same result bit for bit as mhip_integrate.  The partial
+ * sums live in a library-owned static device array of
+ * MHIP_NOALLOC_INTEGRATE_MAX_K x 1024 doubles (512 KiB, held by every device
+ * the library runs on until the process ends); K above
+ * MHIP_NOALLOC_INTEGRATE_MAX_K is rejected with MHIP_ERR_VALUE.  Calls that
+ * overlap on one device share that array and race: order them on one stream.
+ */
+#define MHIP_NOALLOC_INTEGRATE_MAX_K 64
+int mhip_integrate_noalloc(mhip_ctx *ctx, const void *vol, int vol_f64, const void *f, int f_f64, int64_t n, int K,
+                           double *out);
+
+/*
  * GridSurfaceDensity (src/meshoid/grid_deposition.py:123-295), behind
  * Meshoid.SurfaceDensity (src/meshoid/Meshoid.py:595-643): the surface
  * density of f on a res x res grid of sightlines along z, grid_out float64

@@ -19,6 +19,7 @@ ST_AMBIGUOUS, ST_BAND_OVERFLOW, ST_HSML_NOCONV, ST_SINGULAR, ST_COUNT, ST_SEED_M
 ST_RANKDEF = 64
 ST_RADIUS_MISS = 128
 KNN_KEEP_DIST = 1
+NOALLOC_INTEGRATE_MAX_K = 64
 
 EXPORTED = [
     "mhip_version", "mhip_ctx_create", "mhip_ctx_destroy", "mhip_ctx_set_stream",
@@ -31,7 +32,7 @@ EXPORTED = [
     "mhip_batch_create", "mhip_batch_run", "mhip_batch_info", "mhip_batch_destroy",
     "mhip_index_order", "mhip_nearest", "mhip_unique_inverse", "mhip_reconstruct", "mhip_nlist_subset",
     "mhip_surface_density", "mhip_grid_density", "mhip_radtransfer", "mhip_nlist_export_f64",
-    "mhip_nlist_hsml",
+    "mhip_nlist_hsml", "mhip_integrate_noalloc",
 ]
 
 if not os.path.exists(LIB_PATH):
@@ -80,6 +81,7 @@ _SIGS = {
     "mhip_kernel_average": ([_p, _p, _p, _i, _i64, _p], _i),
     "mhip_kernel_variance": ([_p, _p, _p, _i, _i64, _i, _p], _i),
     "mhip_integrate": ([_p, _p, _i, _p, _i, _i64, _i, _p], _i),
+    "mhip_integrate_noalloc": ([_p, _p, _i, _p, _i, _i64, _i, _p], _i),
     "mhip_batch_create": ([_p, _i, _p, _p, _i, _i, _pp], _i),
     "mhip_batch_run": ([_p, _p, _p, _p, _p, _p, _p, _p, _i], _i),
     "mhip_batch_info": ([_p, _p, _p, _p, _p, _p], _i),

@@ -10,6 +10,9 @@
 // neighbour list (column-major sorted-space indices) and recompute each pair
 // distance in float64 from the hi + lo positions, which is exact for float32
 // inputs, instead of reading back float32 distances.
+#include <string>
+
+#include "../../include/meshoid_hip.h"
 #include "mhip_internal.hpp"
 
 namespace mhip {
@@ -240,3 +243,77 @@ int integrate(mhip_ctx *ctx, const void *vol, int vol_f64, const void *f, int f_
 }
 
 }  // namespace mhip
+
+// ---------------------------------------------------------------------------
+// Integrate without allocation: the per-block partial sums of every
+// component live in a file-scope device array instead of the context's
+// scratch.  The array's host-side name is only a stand-in: its device address
+// comes from hipGetSymbolAddress, looked up once per device (with that device
+// current) and kept.  Calls on one device share the array, so calls that
+// overlap on one device race; every partial is written before it is read, so
+// nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+static __device__ __align__(256) double scratch[MHIP_NOALLOC_INTEGRATE_MAX_K * 1024];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+double *noalloc_scratch(mhip_ctx *ctx, int *rc) {
+    *rc = 0;
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) {
+        *rc = mhip::fail(ctx, MHIP_ERR_VALUE, "device index beyond the no-allocation table");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    double *&p = g_scratch_ptr[ctx->device];
+    if (!p) {
+        void *dp = nullptr;
+        hipError_t e = hipGetSymbolAddress(&dp, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !dp) {
+            *rc = mhip::check_hip(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress (scratch)");
+            return nullptr;
+        }
+        p = (double *)dp;
+    }
+    return p;
+}
+}  // namespace
+
+extern "C" int mhip_integrate_noalloc(mhip_ctx *ctx, const void *vol, int vol_f64, const void *f, int f_f64,
+                                      int64_t n, int K, double *out) {
+    using namespace mhip;
+    if (!ctx) return MHIP_ERR_VALUE;
+    if (!vol || !f || !out) return fail(ctx, MHIP_ERR_VALUE, "null argument");
+    if (n < 0 || K < 1) return fail(ctx, MHIP_ERR_VALUE, "bad integrate shape");
+    if (K > MHIP_NOALLOC_INTEGRATE_MAX_K)
+        return fail(ctx, MHIP_ERR_VALUE,
+                    "mhip_integrate_noalloc: at most " + std::to_string(MHIP_NOALLOC_INTEGRATE_MAX_K) +
+                        " components (K = " + std::to_string(K) + ")");
+    int rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (rc) return rc;
+    double *part = noalloc_scratch(ctx, &rc);
+    if (!part) return rc;
+    // the partition of mhip_integrate: the same blocks and sums, bit for bit
+    const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + INT_BLOCK - 1) / INT_BLOCK));
+    ProfRegion pr(ctx, "integrate_noalloc");
+    for (int c = 0; c < K; c++) {
+        double *pc = part + (int64_t)c * 1024;
+#define MHIP_IP(VT, FT)                                                                                    \
+    hipLaunchKernelGGL((k_integrate_partial<VT, FT>), dim3(nb), dim3(INT_BLOCK), 0, ctx->stream, (const VT *)vol, \
+                       (const FT *)f, n, K, c, pc)
+        if (vol_f64) {
+            if (f_f64) MHIP_IP(double, double);
+            else MHIP_IP(double, float);
+        } else {
+            if (f_f64) MHIP_IP(float, double);
+            else MHIP_IP(float, float);
+        }
+#undef MHIP_IP
+        hipLaunchKernelGGL(k_integrate_final, dim3(1), dim3(INT_BLOCK), 0, ctx->stream, pc, nb, out + c);
+    }
+    return check_hip(ctx, hipGetLastError(), "integrate launch");
+}

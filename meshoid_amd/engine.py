@@ -339,6 +339,24 @@ def integrate(vol: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def integrate_noalloc(vol: torch.Tensor, f: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+    """integrate() through mhip_integrate_noalloc, which allocates no device
+    memory (the caller passes out, float64 (K,)); K <= 64 components."""
+    _need(vol, (torch.float32, torch.float64), "vol")
+    _need(f, (torch.float32, torch.float64), "f")
+    _need(out, (torch.float64,), "out")
+    f2 = f.reshape(f.shape[0], -1)
+    if f2.shape[0] != vol.shape[0]:
+        raise ValueError("f and vol must have the same number of rows")
+    K = f2.shape[1]
+    if out.numel() != K:
+        raise ValueError("out must hold one value per component")
+    ctx = context(f.device)
+    raise_for(lib.mhip_integrate_noalloc(ctx, ptr(vol), int(vol.dtype == torch.float64), ptr(f2),
+                                         int(f.dtype == torch.float64), f2.shape[0], K, ptr(out)), ctx)
+    return out
+
+
 def nearest(index: SpatialIndex, points: torch.Tensor) -> torch.Tensor:
     """Nearest particle of each point, cKDTree.query(points)[1]
     (src/meshoid/Meshoid.py:409): points (nt, dim) float64 device tensor;
