@@ -5,7 +5,7 @@ Workload: 2^24 clustered particles (256 Gaussian clumps, sigma log-uniform in
 [0.05, 5], clump masses dN/dM ~ M^-2, 10% uniform background, 100-unit box,
 float32 positions), des_ngb = 32, one float32 field f = sin(kx) sin(ky) sin(kz)
 with k = 2 pi / 10.  One step = the whole path on device-resident inputs:
-spatial index build (Morton radix sort + tree), exact kNN + smoothing length +
+spatial index build (Hilbert-key radix sort + tree), exact kNN + smoothing length +
 density, and the least-squares gradient of f.
 
 Prints ONE JSON line (the metric contract), with

@@ -83,8 +83,8 @@ int mhip_index_destroy(mhip_index *ix);
 /* host-side copy of n, dim, boxsize and the bounding box (lo xyz, hi xyz) */
 int mhip_index_info(const mhip_index *ix, int64_t *n, int *dim, double *boxsize, float *bbox6);
 /* the index's particle order: perm_out uint32[n] (device), the original row
- * of each sorted position (Morton order of the positions in the cubic frame
- * of the bounding box or periodic box, 21 bits per axis, stable) */
+ * of each sorted position (Hilbert order of the positions' cells in the cubic
+ * frame of the bounding box or periodic box, 21 bits per axis, stable) */
 int mhip_index_order(mhip_ctx *ctx, const mhip_index *ix, uint32_t *perm_out);
 
 /*

@@ -1,7 +1,7 @@
 // Shared device helpers and internal host structures of libmeshoid_hip.
 //
 // Conventions used by every kernel in this library:
-//   * particles are reordered once into Morton order ("sorted space"); positions
+//   * particles are reordered once into Hilbert order ("sorted space"); positions
 //     live there as float32 structure-of-arrays sx/sy/sz padded to a multiple of
 //     LEAF with NaN coordinates (every distance comparison against a padding
 //     slot is false, also after a periodic wrap).  Float64 inputs keep a float32 residual
@@ -198,7 +198,7 @@ struct mhip_index {
     float *nodes = nullptr;
     int64_t node_cap = 0;
     float bbox[6] = {};
-    float frame[4] = {};  // Morton quantisation: origin xyz, scale (cells per unit, 2^21 - 1 over the extent)
+    float frame[4] = {};  // sort-key quantisation: origin xyz, scale (cells per unit, 2^21 - 1 over the extent)
 };
 
 struct mhip_nlist {

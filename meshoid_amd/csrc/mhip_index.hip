@@ -1,6 +1,7 @@
 // Spatial index: the GPU replacement of scipy's cKDTree build
-// (src/meshoid/Meshoid.py:142-145).  Particles are sorted by a 21-bit-per-axis
-// Morton key with a project-written LSD radix sort and reordered into float32
+// (src/meshoid/Meshoid.py:142-145).  Particles are sorted by the Hilbert index
+// of their cell on a 2^21-per-axis grid (a project-written LSD radix sort) and
+// reordered into float32
 // structure-of-arrays; an implicit tree groups LEAF consecutive particles into
 // leaves and FANOUT consecutive nodes into parents, each with its bounding box.
 #include <algorithm>
@@ -91,22 +92,18 @@ __global__ __launch_bounds__(256) void k_bbox_final(const float *__restrict__ pa
 }
 
 // ---------------------------------------------------------------------------
-// Morton keys (21 bits per axis, x in bit 0)
+// sort keys: Hilbert index of the particle's cell (21 bits per axis)
 // ---------------------------------------------------------------------------
 template <typename PT>
-__global__ __launch_bounds__(256) void k_morton(const PT *__restrict__ pos, int64_t n, int dim,
-                                               const float *__restrict__ bb,
-                                               uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+__global__ __launch_bounds__(256) void k_sfc_keys(const PT *__restrict__ pos, int64_t n, int dim,
+                                                 const float *__restrict__ bb,
+                                                 uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float scale = bb[11];
-    uint64_t key = 0;
-    for (int c = 0; c < dim; c++) {
-        float t = ((float)pos[i * dim + c] - bb[8 + c]) * scale;
-        t = fminf(fmaxf(t, 0.f), 2097151.0f);
-        key |= spread21((uint64_t)t) << c;
-    }
-    keys[i] = key;
+    const float scale = bb[11];
+    uint32_t c[3] = {0u, 0u, 0u};
+    for (int a = 0; a < dim; a++) c[a] = key_cell((float)pos[i * dim + a], bb[8 + a], scale);
+    keys[i] = hilbert_key(c[0], c[1], c[2], dim);
     vals[i] = (uint32_t)i;
 }
 
@@ -442,16 +439,16 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     }
     hipLaunchKernelGGL(k_bbox_final, dim3(1), b256, 0, ctx->stream, part, nb, dim, (float)box, bb);
     pr_bb.end();
-    ProfRegion pr_key(ctx, "index_morton_sort");
+    ProfRegion pr_key(ctx, "index_key_sort");
     if (pos_f64)
-        hipLaunchKernelGGL(k_morton<double>, g1, b256, 0, ctx->stream, (const double *)pos, n, dim, bb,
+        hipLaunchKernelGGL(k_sfc_keys<double>, g1, b256, 0, ctx->stream, (const double *)pos, n, dim, bb,
                            k0, v0);
     else
-        hipLaunchKernelGGL(k_morton<float>, g1, b256, 0, ctx->stream, (const float *)pos, n, dim, bb,
+        hipLaunchKernelGGL(k_sfc_keys<float>, g1, b256, 0, ctx->stream, (const float *)pos, n, dim, bb,
                            k0, v0);
     uint64_t *ks;
     uint32_t *vs;
-    if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, 21 * dim, &ks, &vs))) return rc;
+    if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, KEY_BITS_PER_AXIS * dim, &ks, &vs))) return rc;
     pr_key.end();
     ProfRegion pr_g(ctx, "index_reorder_boxes");
     const dim3 gp((unsigned)((npad + 255) / 256));

@@ -1,6 +1,7 @@
 // Internal (non-exported) launch helpers shared by the .hip translation units.
 #pragma once
 #include "mhip_common.hpp"
+#include "mhip_sfc.hpp"
 
 namespace mhip {
 
@@ -9,18 +10,6 @@ namespace mhip {
 int fail(mhip_ctx *ctx, int code, const std::string &msg);
 int check_hip(mhip_ctx *ctx, hipError_t e, const char *what);
 void *scratch(mhip_ctx *ctx, size_t bytes, int *rc);
-
-// the 21 low bits of x spread to every third bit (one axis of a 63-bit
-// Morton key)
-MHIP_DEV inline uint64_t spread21(uint64_t x) {
-    x &= 0x1fffffull;
-    x = (x | x << 32) & 0x1f00000000ffffull;
-    x = (x | x << 16) & 0x1f0000ff0000ffull;
-    x = (x | x << 8) & 0x100f00f00f00f00full;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
-    x = (x | x << 2) & 0x1249249249249249ull;
-    return x;
-}
 
 // LSD radix sort of (u64 key, u32 value) pairs on bits [0, bits) of the key.
 // Buffers: k0/v0 hold the input; the sorted result is returned in *kout/*vout

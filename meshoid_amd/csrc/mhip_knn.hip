@@ -6,7 +6,7 @@
 //                                                src/meshoid/kernel_density.py:14-57
 //   density = des_ngb*m/(volnorm*h^dim)          src/meshoid/Meshoid.py:169-170
 //
-// Design (one lane per query; 64 consecutive queries in Morton order per
+// Design (one lane per query; 64 consecutive queries in Hilbert order per
 // wavefront; the tree is a complete 8-ary tree in heap numbering, root 0,
 // children of x at 8x+1..8x+8, leaves of 16 particles on the last level):
 //   main    Every lane keeps the k+1 smallest float32 squared distances of
@@ -1006,7 +1006,7 @@ static int nl_reserve(mhip_ctx *ctx, mhip_nlist *nl, int64_t nq, int k) {
     return 0;
 }
 
-// slots of the query rows qidx in sorted (Morton) order: qsorted[s] is the
+// slots of the query rows qidx in sorted (Hilbert) order: qsorted[s] is the
 // sorted-space particle of slot s, qrow[s] its row in qidx
 static int query_setup(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq, mhip_nlist *nl) {
     ProfRegion pr(ctx, "knn_query_setup");
@@ -1063,7 +1063,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     nl->k = k;
     nl->index = ix;
     nl->has_d2 = (flags & 1) != 0;
-    // query slots in Morton order
+    // query slots in Hilbert order
     const bool masked = qidx != nullptr;
     nl->masked = masked;
     if (masked && (rc = query_setup(ctx, ix, qidx, nq, nl))) return rc;
@@ -1354,10 +1354,10 @@ int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *d
 // ---------------------------------------------------------------------------
 // nearest particle of arbitrary float64 points: cKDTree.query(x)[1], k = 1
 // (src/meshoid/Meshoid.py:409).  The points are first put in the index's
-// Morton order, so that a wavefront's 64 lanes serve a compact block of
+// Hilbert order, so that a wavefront's 64 lanes serve a compact block of
 // points.  (1) Each lane finds, by binary search over the leaves' first
-// Morton keys, the leaf where its point's key falls, and scans it and its
-// two neighbours (in Morton order these usually hold the nearest particle):
+// sort keys, the leaf where its point's key falls, and scans it and its
+// two neighbours (in Hilbert order these usually hold the nearest particle):
 // every lane starts with a tight bound.  (2) A wave-uniform walk, inside out
 // from the middle lane's leaf (its siblings, then the rest of the
 // grandparent's subtree, ... up to the root), tests each node once for all
@@ -1371,13 +1371,13 @@ int nlist_export(mhip_ctx *ctx, const mhip_nlist *nl, int32_t *ngb_out, float *d
 // ---------------------------------------------------------------------------
 struct NearestArgs {
     TreeView t;
-    const uint32_t *order;  // point of each thread (Morton order), null = identity
+    const uint32_t *order;  // point of each thread (Hilbert order), null = identity
     const uint32_t *perm;
     const void *pos_in;
     int64_t n;
     int dim;
     double box;        // > 0: periodic in [0, box)
-    float org[3], scale;  // the index's Morton quantisation frame
+    float org[3], scale;  // the index's key quantisation frame
     const double *tg;  // (nt, dim) row-major
     int64_t nt;
     int64_t *out;      // original particle index per point, -1 for a non-finite point
@@ -1395,7 +1395,7 @@ MHIP_DEV double exact_d2_point(const PT *pos, int dim, uint32_t j, const double 
     return s;
 }
 
-// Morton key of each point in the index's quantisation frame (as the
+// sort key of each point in the index's quantisation frame (as the
 // particles' keys are formed), periodic points wrapped into the box first
 __global__ __launch_bounds__(256) void k_point_keys(const double *__restrict__ tg, int64_t nt, int dim,
                                                     double box, float ox, float oy, float oz, float scale,
@@ -1403,28 +1403,22 @@ __global__ __launch_bounds__(256) void k_point_keys(const double *__restrict__ t
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= nt) return;
     const float org[3] = {ox, oy, oz};
-    uint64_t key = 0;
+    uint32_t cc[3] = {0u, 0u, 0u};
     for (int c = 0; c < dim; c++) {
         double v = tg[i * dim + c];
         if (box > 0.0 && (v < 0.0 || v >= box)) v -= floor(v / box) * box;
-        float t = ((float)v - org[c]) * scale;
-        t = fminf(fmaxf(t, 0.f), 2097151.0f);  // NaN -> 0
-        key |= spread21((uint64_t)t) << c;
+        cc[c] = key_cell((float)v, org[c], scale);
     }
-    keys[i] = key;
+    keys[i] = hilbert_key(cc[0], cc[1], cc[2], dim);
     vals[i] = (uint32_t)i;
 }
 
-// Morton key of sorted particle p, formed as the index build forms it
+// sort key of sorted particle p, formed as the index build forms it
 MHIP_DEV uint64_t particle_key(const TreeView &t, const float (&org)[3], float scale, int dim, int64_t p) {
     const float v[3] = {t.sx[p], t.sy[p], t.sz[p]};
-    uint64_t key = 0;
-    for (int c = 0; c < dim; c++) {
-        float u = (v[c] - org[c]) * scale;
-        u = fminf(fmaxf(u, 0.f), 2097151.0f);
-        key |= spread21((uint64_t)u) << c;
-    }
-    return key;
+    uint32_t cc[3] = {0u, 0u, 0u};
+    for (int c = 0; c < dim; c++) cc[c] = key_cell(v[c], org[c], scale);
+    return hilbert_key(cc[0], cc[1], cc[2], dim);
 }
 
 template <bool PER, bool P64, typename PT>
@@ -1469,12 +1463,9 @@ __global__ __launch_bounds__(256) void k_nearest(NearestArgs a) {
     // point's key; it and its neighbours give the lane a first bound
     uint64_t kq = 0;
     {
-        const double v[3] = {tp[0], tp[1], tp[2]};
-        for (int c = 0; c < a.dim; c++) {
-            float u = ((float)v[c] - a.org[c]) * a.scale;
-            u = fminf(fmaxf(u, 0.f), 2097151.0f);  // NaN -> 0
-            kq |= spread21((uint64_t)u) << c;
-        }
+        uint32_t cc[3] = {0u, 0u, 0u};
+        for (int c = 0; c < a.dim; c++) cc[c] = key_cell((float)tp[c], a.org[c], a.scale);
+        kq = hilbert_key(cc[0], cc[1], cc[2], a.dim);
     }
     uint32_t lo = 0, hi = (uint32_t)t.nleaf;
     while (hi - lo > 1) {
@@ -1626,7 +1617,7 @@ int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int6
     const dim3 g((unsigned)((nt + 255) / 256));
     hipStream_t s = ctx->stream;
     if (nt >= 4096 && nt <= 0xffffffffll) {
-        // points in the index's Morton order
+        // points in the index's Hilbert order
         ProfRegion pk(ctx, "nearest_sort");
         int rc = 0;
         const size_t cw = radix_counts_words(nt);
@@ -1638,7 +1629,7 @@ int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int6
                            ix->frame[2], ix->frame[3], k0, v0);
         uint64_t *ks;
         uint32_t *vs;
-        if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nt, 21 * dim, &ks, &vs))) return rc;
+        if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nt, KEY_BITS_PER_AXIS * dim, &ks, &vs))) return rc;
         a.order = vs;
     }
     ProfRegion pr(ctx, "nearest");

@@ -3,8 +3,8 @@
 // [r_1, c_d r_{k-1}] of src/meshoid/kernel_density.py:37-38).
 //
 //   features  28 per particle: log(1 + count) of the 3x3x3 block of coarse
-//             cells around the particle's cell (cell = 1/128 of the Morton
-//             quantisation frame, so coarse cells are Morton prefixes and the
+//             cells around the particle's cell (cell = 1/128 of the sort-key
+//             quantisation frame, so coarse cells are Hilbert-key prefixes and the
 //             counts are run lengths of the sorted order) + log(cell size).
 //   network   28 -> 64 -> 64 -> 1, SiLU; bfloat16 weights and activations,
 //             float32 accumulation.  Output: log(h / cell).
@@ -52,7 +52,7 @@ struct SeedView {
     const uint32_t *counts;  // [G^3] particles per coarse cell
 };
 
-// coarse cell of sorted particle p: the same arithmetic as the Morton keys
+// coarse cell of sorted particle p: the same arithmetic as the sort keys
 MHIP_DEV void coarse_cell(const SeedView &v, int64_t p, int (&c)[3]) {
     const float x[3] = {v.sx[p], v.sy[p], v.sz[p]};
 #pragma unroll
@@ -97,7 +97,7 @@ __global__ void k_fill_u32(uint32_t *__restrict__ a, int64_t n, uint32_t val) {
     if (i < n) a[i] = val;
 }
 
-// particles per coarse cell: a cell is a Morton prefix, so its particles are a
+// particles per coarse cell: a cell is a Hilbert-key prefix, so its particles are a
 // contiguous run of the sorted order; each wavefront adds its runs' lengths
 __global__ __launch_bounds__(256) void k_seed_hist(SeedView v, uint32_t *__restrict__ counts) {
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;

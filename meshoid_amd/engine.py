@@ -73,7 +73,7 @@ def _need(t, dtypes, name):
 
 
 class SpatialIndex:
-    """Morton-sorted particle set + implicit tree (the cKDTree replacement)."""
+    """Hilbert-sorted particle set + implicit tree (the cKDTree replacement)."""
 
     def __init__(self):
         self.h = ctypes.c_void_p()
@@ -97,7 +97,7 @@ class SpatialIndex:
         return self
 
     def order(self) -> torch.Tensor:
-        """Original row of each sorted position (int64, Morton order)."""
+        """Original row of each sorted position (int64, Hilbert order)."""
         perm = torch.empty(self.n, dtype=torch.int32, device=self.pos.device)
         ctx = context(self.pos.device)
         raise_for(lib.mhip_index_order(ctx, self.h, ptr(perm)), ctx)
