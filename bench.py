@@ -46,8 +46,6 @@ def alg_bytes(region: str, k: int, K: int) -> float:
         return 12 + 12 * k + 16 + 4 * k
     if region == "knn_pass1":  # own pos + k neighbour positions + h, rho
         return 12 + 12 * k + 8
-    if region == "knn_pass2":  # same gathers again: neighbour list collection
-        return 12 + 12 * k
     if region in ("apply_D1", "apply_D2"):  # own pos, h, field; k x (pos, field); grad
         return 12 + 4 + 4 * K + k * (12 + 4 * K) + 12 * K
     return 0.0

@@ -40,6 +40,9 @@ extern "C" {
 
 /* mhip_knn_hsml flags */
 #define MHIP_KNN_KEEP_DIST 1 /* keep squared distances (needed by mhip_nlist_export) */
+#define MHIP_KNN_SMALL_RECORD 2 /* test hook: cap each wavefront's leaf record at 8 entries,
+                                  * so that every wavefront collects its members by the
+                                  * overflow path (a second walk); results are unchanged */
 
 typedef struct mhip_ctx mhip_ctx;
 typedef struct mhip_index mhip_index;

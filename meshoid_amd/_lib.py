@@ -19,6 +19,7 @@ ST_AMBIGUOUS, ST_BAND_OVERFLOW, ST_HSML_NOCONV, ST_SINGULAR, ST_COUNT, ST_SEED_M
 ST_RANKDEF = 64
 ST_RADIUS_MISS = 128
 KNN_KEEP_DIST = 1
+KNN_SMALL_RECORD = 2  # test hook: every wavefront takes the leaf-record overflow path
 NOALLOC_INTEGRATE_MAX_K = 64
 
 EXPORTED = [

@@ -136,7 +136,7 @@ int mhip_prof_read(mhip_ctx *ctx, char *buf, int64_t buflen) {
                                                      "stat_lane_need", "stat_maxlane_need",
                                                      "stat_parent_max", "stat_nn_waves", "stat_nn_nodes",
                                                      "stat_nn_leaves", "stat_nn_exact", "stat_newton",
-                                                     "stat_newton_wmax"};
+                                                     "stat_newton_wmax", "stat_overflow_waves"};
         unsigned long long hs[16];
         rc = check_hip(ctx, hipMemcpy(hs, ctx->stats, sizeof hs, hipMemcpyDeviceToHost), "stats read");
         if (rc) return rc;

@@ -141,6 +141,7 @@ enum : int {
     STAT_NN_EXACT,        // points re-ranked in float64
     STAT_NEWTON,          // smoothing-length iterations (sum over query lanes)
     STAT_NEWTON_WMAX,     // per wavefront, the largest iteration count of a lane (sum)
+    STAT_OVERFLOW_WAVES,  // wavefronts whose leaf record overflowed (members by a second walk)
     STAT_COUNT
 };
 static_assert(STAT_COUNT <= 16, "the work-counter buffer holds 16 entries");

@@ -22,18 +22,20 @@
 //           the lane's (k+1)-th distance.  Queues drain into the sorted lists
 //           in uniform counted loops.  Leaves where some lane queued a
 //           candidate are recorded.  (4) Epilogue: smoothing length
-//           (safeguarded Newton, compensated float32 sums), density, volume,
+//           (safeguarded float32 Newton, then one float64 Newton step over
+//           the final members), density, volume,
 //           and the member set {d2 <= tau}: each lane tests the recorded
 //           leaves against its own final radius and scans only those, one
 //           leaf per lane per step (per-lane loads); the indices are staged
 //           in LDS and written as whole rows.  Queries whose k-th and
 //           (k+1)-th distances are closer than float32 rounding can order
-//           are flagged.
-//   pass2   fallback for waves whose leaf record overflowed: per-lane
-//           traversal with the final radius, collecting the members.
+//           are flagged.  A wavefront whose leaf record overflows walks
+//           the tree again with every lane's final radius and collects the
+//           members in batches of recorded leaves.
 //   resolve for flagged queries only: members in the rounding band around tau
 //           are re-ranked with float64 distances from the caller's positions,
 //           so neighbour sets equal the float64 kNN except for float64 ties.
+#include "../../include/meshoid_hip.h"
 #include "mhip_internal.hpp"
 
 namespace mhip {
@@ -48,7 +50,6 @@ constexpr float BAND_REL = 16.0f * U32;      // resolver band half-width
 constexpr float PRUNE_REL = 1.0f + 1e-5f;    // box-test safety factor
 constexpr int BAND_CAP = 32;
 constexpr int64_t RESOLVE_WAVES = 8192;      // resolver grid (one-lane wavefronts first)
-constexpr int32_t ST_REDO = 1 << 16;          // internal: members collected by pass2
 
 struct TreeView {
     const float *sx, *sy, *sz, *lx, *ly, *lz;
@@ -212,6 +213,7 @@ struct KnnArgs {
     float *nl_d2;
     unsigned long long *stats;  // work counters, may be null
     uint32_t *amb;              // [0] count, [1..] ambiguous slots
+    uint32_t wlcap;             // leaf-record capacity used (<= WLCAP; smaller only as a test hook)
 };
 
 MHIP_DEV int64_t slot_of_lane(int64_t *first) {
@@ -219,29 +221,6 @@ MHIP_DEV int64_t slot_of_lane(int64_t *first) {
     const int64_t wave0 = ((int64_t)b * KNN_WAVES + (threadIdx.x >> 6)) * WAVE;
     *first = wave0;
     return wave0 + (threadIdx.x & (WAVE - 1));
-}
-
-// collects the members {d2 < tau} plus ties d2 == tau up to k of one leaf
-template <bool PER, bool P64>
-MHIP_DEV void collect_leaf(const KnnArgs &a, const QPos &q, int64_t leaf, float tau, int &eq_left,
-                           int &cnt, int64_t slot) {
-    float cx[LEAF], cy[LEAF], cz[LEAF], clx[LEAF], cly[LEAF], clz[LEAF];
-    load_leaf<P64>(a.t, leaf, cx, cy, cz, clx, cly, clz);
-    const int64_t base = leaf * LEAF;
-#pragma unroll
-    for (int i = 0; i < LEAF; i++) {
-        const float d2 = dist2<PER, P64>(a.t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
-        bool take = d2 < tau;
-        if (d2 == tau && eq_left > 0) {
-            take = true;
-            eq_left--;
-        }
-        if (take && cnt < a.k) {
-            a.nl_idx[(int64_t)cnt * a.nq + slot] = (uint32_t)(base + i);
-            if (a.nl_d2) a.nl_d2[(int64_t)cnt * a.nq + slot] = d2;
-            cnt++;
-        }
-    }
 }
 
 MHIP_DEV void pad_list(const KnnArgs &a, int cnt, uint32_t qp, int64_t slot) {
@@ -363,13 +342,15 @@ MHIP_DEV float hsml_solve(float (&A)[KCAP], int k, int dim, float seed, int32_t 
         const float dndh = -norm * ds * ih;
         const float step = f / dndh;
         // n(h) is C^2 (W is C^2 at q = 1/2 and q = 1), so Newton converges
-        // quadratically: after a step below 1e-4 h the error of h - step is
-        // at the float32 noise floor.  Such a step ends the solve even when
-        // it rounds onto the bracket end that h itself has just become
-        // (otherwise a converged lane would bisect the whole bracket again
-        // and hold its wavefront for ~15 more iterations).  A collapsed
-        // bracket also ends the solve.
-        if (fabsf(step) <= 1e-4f * h) {
+        // quadratically: after a step below 3e-4 h the error of h - step is
+        // at the float32 noise floor (~3e-7 h with the compensated sums), and
+        // the float64 Newton step over the final members that follows
+        // (refine_h64) takes it to ~2e-13 h.  Such a step ends the
+        // solve even when it rounds onto the bracket end that h itself has
+        // just become (otherwise a converged lane would bisect the whole
+        // bracket again and hold its wavefront for ~15 more iterations).  A
+        // collapsed bracket also ends the solve.
+        if (fabsf(step) <= 3e-4f * h) {
             h = fminf(fmaxf(h - step, lo), hi);
             break;
         }
@@ -625,7 +606,7 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
                 qn += d2 < t1;
             }
             if (__any(qn > before)) {
-                if (nwl < WLCAP && lane == 0) s_wl[w][nwl] = (uint32_t)li;
+                if (nwl < a.wlcap && lane == 0) s_wl[w][nwl] = (uint32_t)li;
                 nwl++;
             }
             if (__any(qn > QCAP - LEAF)) break;
@@ -696,7 +677,7 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     // members: a candidate at or below tau passed its lane's queue filter
     // when its leaf was scanned, so it lies in a recorded leaf; leaves no
     // final ball reaches are skipped
-    if (nwl <= WLCAP) {
+    {
         // Unless the row is ambiguous, exactly k particles have d2 <= tau
         // (the (k+1)-th distance is strictly larger), so "d2 <= tau" selects
         // the members.  Ambiguous rows (float32 ties at tau included) are
@@ -708,8 +689,9 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         int cnt = 0;
         uint32_t nscan = 0;
         const float tb = tau * PRUNE_REL;
-        for (uint32_t r0 = 0; r0 < nwl; r0 += 64) {
-            const uint32_t nr = min(64u, nwl - r0);
+        auto collect = [&](uint32_t nrec) {
+        for (uint32_t r0 = 0; r0 < nrec; r0 += 64) {
+            const uint32_t nr = min(64u, nrec - r0);
             uint32_t nlo = 0, nhi = 0;
             for (uint32_t r = 0; r < nr; r++) {
                 const uint32_t li = uniform(s_wl[w][r0 + r]);
@@ -745,6 +727,48 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
                 }
             }
         }
+        };
+        if (nwl <= a.wlcap) {
+            collect(nwl);
+        } else {
+            // the leaf record overflowed (a wavefront whose queries spread
+            // over a large region): walk the tree again, pruned by every
+            // lane's final radius, and collect in batches of wlcap leaves
+            MaskStack ws;
+            int wl = 0;
+            uint32_t wx = 0, nrec = 0;
+            ws.put(0, expand_mask<PER>(t, qb, 0u, tb));
+            while (true) {
+                const uint32_t m = ws.get(wl);
+                if (m == 0u) {
+                    if (wl == 0) break;
+                    wl--;
+                    wx = (wx - 1) >> 3;
+                    continue;
+                }
+                const uint32_t c = (uint32_t)__builtin_ctz(m);
+                ws.clear(wl, c);
+                const uint32_t y = FANOUT * wx + 1 + c;
+                if (wl + 1 < depth) {
+                    wl++;
+                    wx = y;
+                    ws.put(wl, expand_mask<PER>(t, qb, y, tb));
+                    continue;
+                }
+                const int64_t cand = (int64_t)(y - first_leaf);
+                if (cand >= t.nleaf) continue;
+                if (lane == 0) s_wl[w][nrec] = (uint32_t)cand;
+                if (++nrec == a.wlcap) {
+                    __builtin_amdgcn_wave_barrier();
+                    collect(nrec);
+                    nrec = 0;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            collect(nrec);
+            if (a.stats && lane == 0) atomicAdd(a.stats + STAT_OVERFLOW_WAVES, 1ull);
+        }
         if (a.stats && lane == 0) atomicAdd(a.stats + STAT_P2_NODES, (unsigned long long)nscan);
         if (STAGE && live) {
             for (int j = 0; j < a.k; j++) a.nl_idx[(int64_t)j * a.nq + slot] = j < cnt ? s_buf[w][j][lane] : qp;
@@ -764,8 +788,6 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
             else
                 h64 = refine_h64<PER, P64>(a, q, h64, [&](int j) { return a.nl_idx[(int64_t)j * a.nq + slot]; });
         }
-    } else {
-        st |= ST_REDO;
     }
     if (live) {
         write_hsml(a, slot, row, h64);
@@ -783,45 +805,6 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         base = __shfl(base, leader);
         if (amb) a.amb[1 + base + __popcll(am & ((1ull << lane) - 1ull))] = (uint32_t)slot;
     }
-}
-
-// members for lanes whose leaf record overflowed: traversal with radius tau
-template <bool PER, bool P64>
-__global__ __launch_bounds__(256) void k_knn_pass2(KnnArgs a) {
-    const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (slot >= a.nq) return;
-    int32_t st = a.st_slot[slot];
-    if (!(st & ST_REDO)) return;
-    const TreeView &t = a.t;
-    const uint32_t qp = a.qsorted ? a.qsorted[slot] : (uint32_t)slot;
-    const QPos q = load_q<P64>(t, qp);
-    const float tau = a.tau[slot];
-    int eq_left = a.k - a.nlt[slot], cnt = 0;
-    const uint32_t first_leaf = (uint32_t)t.first_leaf;
-    uint32_t x = 0;
-    bool more = true;
-    while (more) {
-        if (node_d2<PER>(t, q, x) <= tau * PRUNE_REL) {
-            if (x < first_leaf) {
-                x = FANOUT * x + 1;
-                continue;
-            }
-            if (x - first_leaf < t.nleaf)
-                collect_leaf<PER, P64>(a, q, x - first_leaf, tau, eq_left, cnt, slot);
-        }
-        more = next_after(x);
-    }
-    st &= ~ST_REDO;
-    if (cnt != a.k) {
-        st |= ST_COUNT;
-        pad_list(a, cnt, qp, slot);
-    } else if (a.h_slot[slot] > 0.f && !(st & ST_HSML_NOCONV)) {
-        const uint32_t row = a.qrow ? a.qrow[slot] : a.perm[slot];
-        const double h64 = refine_h64<PER, P64>(a, q, (double)a.h_slot[slot],
-                                                [&](int j) { return a.nl_idx[(int64_t)j * a.nq + slot]; });
-        write_hsml(a, slot, row, h64);
-    }
-    a.st_slot[slot] = st;
 }
 
 // float64 squared distance between original particles i and j
@@ -1094,6 +1077,7 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     a.nl_d2 = nl->has_d2 ? nl->d2 : nullptr;
     a.stats = ctx->prof >= 2 ? ctx->stats : nullptr;
     a.amb = nl->amb;
+    a.wlcap = (flags & MHIP_KNN_SMALL_RECORD) ? 8u : (uint32_t)WLCAP;
     const bool p64 = ix->pos_f64 != 0;
     const dim3 g((unsigned)((nq + KNN_BLOCK - 1) / KNN_BLOCK));
     hipStream_t s = ctx->stream;
@@ -1138,20 +1122,6 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
             launch_main<129>(r, per, p64, g, s);
     }
     pr1.end();
-    ProfRegion pr2(ctx, "knn_pass2");
-    const dim3 g2((unsigned)((nq + 255) / 256));
-    if (per) {
-        if (p64)
-            hipLaunchKernelGGL((k_knn_pass2<true, true>), g2, dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_knn_pass2<true, false>), g2, dim3(256), 0, s, a);
-    } else {
-        if (p64)
-            hipLaunchKernelGGL((k_knn_pass2<false, true>), g2, dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_knn_pass2<false, false>), g2, dim3(256), 0, s, a);
-    }
-    pr2.end();
     ProfRegion pr3(ctx, "knn_resolve");
     const dim3 gr((unsigned)std::min<int64_t>(RESOLVE_WAVES, (nq + 63) / 64));
     if (per) {

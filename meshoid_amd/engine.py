@@ -154,10 +154,12 @@ class NeighborList:
 
 
 def knn_hsml(index: SpatialIndex, k: int, qidx=None, m_rows=None, seed_h=None, keep_dist=True,
-             nlist: NeighborList | None = None, out=None, seed_r=None):
+             nlist: NeighborList | None = None, out=None, seed_r=None, small_record=False):
     """kNN + smoothing length + density.  Returns dict(h, rho, vol, status, nlist).
     seed_h: Newton seed per row; seed_r: search-radius bound per row (rows where
-    it is too small are recomputed without it).  Neither changes the result."""
+    it is too small are recomputed without it).  Neither changes the result.
+    small_record: test hook, every wavefront collects its members by the
+    leaf-record overflow path (results unchanged)."""
     dev = index.pos.device
     _need(qidx, (torch.int64,), "qidx")
     _need(m_rows, (torch.float64,), "m_rows")
@@ -178,7 +180,9 @@ def knn_hsml(index: SpatialIndex, k: int, qidx=None, m_rows=None, seed_h=None, k
     ctx = context(dev)
     rc = lib.mhip_knn_hsml(ctx, index.h, ptr(qidx), nq, int(k), ptr(m_rows), ptr(seed_h),
                            ptr(seed_r), ptr(out.get("h")), ptr(out.get("rho")), ptr(out.get("vol")),
-                           ptr(out.get("status")), _lib.KNN_KEEP_DIST if keep_dist else 0, nl.h)
+                           ptr(out.get("status")),
+                           (_lib.KNN_KEEP_DIST if keep_dist else 0) | (_lib.KNN_SMALL_RECORD if small_record else 0),
+                           nl.h)
     raise_for(rc, ctx)
     nl.nq, nl.k, nl.index, nl.keep_dist = nq, int(k), index, keep_dist
     out["nlist"] = nl

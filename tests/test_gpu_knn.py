@@ -199,3 +199,43 @@ def test_knn_large_clustered_invariants():
     say("max |n(h) - 32| with float64 distances and h:", err)
     assert err < 1e-9
     say("ambiguous rows resolved in fp64:", int(((st & 1) != 0).sum()))
+
+
+@pytest.mark.parametrize("case", ["clustered_f32", "periodic_f64", "masked", "k64"])
+def test_leaf_record_overflow_path_matches(case):
+    """Wavefronts whose leaf record overflows collect their members by a
+    second tree walk with the final radii.  The test hook caps the record at
+    8 leaves, so every wavefront takes that path; neighbour sets, distances,
+    smoothing lengths and densities must equal the normal path's."""
+    torch = _torch()
+    from meshoid_amd import engine, synthetic
+
+    say("overflow path", case)
+    k = 64 if case == "k64" else 32
+    box = None
+    if case == "periodic_f64":
+        pos = np.random.default_rng(3).random((60000, 3)) * 10.0
+        box = 10.0
+    else:
+        pos = synthetic.clustered(200000, seed=11, nclump=32, box=10.0, background=0.1)
+    qidx = None
+    if case == "masked":
+        qidx = np.sort(np.random.default_rng(5).choice(len(pos), 30000, replace=False))
+    p = torch.from_numpy(np.ascontiguousarray(pos)).cuda()
+    ix = engine.SpatialIndex().build(p, box)
+    q = None if qidx is None else torch.from_numpy(qidx.astype(np.int64)).cuda()
+    res = []
+    for small in (False, True):
+        out = engine.knn_hsml(ix, k, qidx=q, keep_dist=True, small_record=small)
+        ngb, dist = out["nlist"].export()
+        h64 = out["nlist"].h64()
+        torch.cuda.synchronize()
+        res.append((ngb.cpu().numpy(), dist.cpu().numpy(), h64.cpu().numpy(), out["rho"].cpu().numpy(),
+                    out["status"].cpu().numpy()))
+    (n0, d0, h0, r0, s0), (n1, d1, h1, r1, s1) = res
+    assert np.array_equal(n0, n1)
+    assert np.array_equal(d0, d1)
+    assert np.array_equal(s0, s1)
+    # the float64 refinement sums the same members in another order
+    np.testing.assert_allclose(h1, h0, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(r1, r0, rtol=3e-7, atol=0)
