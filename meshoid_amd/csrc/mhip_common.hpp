@@ -207,6 +207,7 @@ struct mhip_nlist {
     int k = 0;
     uint32_t *qsorted = nullptr;  // sorted-space particle index of each query slot
     uint32_t *qrow = nullptr;     // output row of each query slot
+    uint32_t *rinv = nullptr;     // query slot of each output row (masked lists)
     uint32_t *idx = nullptr;      // column-major [k][nq] sorted-space neighbour indices
     float *d2 = nullptr;          // column-major [k][nq] squared distances
     float *tau = nullptr;         // k-th squared distance per slot

@@ -13,6 +13,8 @@
 // field, then solves with an unpivoted float64 LDL^T (E is symmetric positive
 // semi-definite; an exactly zero or NaN pivot is reported as singular, as
 // np.linalg.inv raises on an exactly zero pivot).
+#include <type_traits>
+
 #include "mhip_internal.hpp"
 #include "mhip_linalg.hpp"
 
@@ -23,22 +25,55 @@ namespace mhip {
 // ---------------------------------------------------------------------------
 
 
-// float64 offset of sorted particle p from the host particle (qx, qy, qz):
-// exact for float32 positions, minimum image when periodic
-MHIP_DEV void pair_offset(const ApplyArgs &a, uint32_t p, double qx, double qy, double qz, double (&d)[3]) {
-    d[0] = (double)a.sx[p] - qx;
-    d[1] = (double)a.sy[p] - qy;
-    d[2] = (double)a.sz[p] - qz;
+// float64 offset of a neighbour (float32 position c, float32 residual cl of
+// float64 inputs) from the host particle (qx, qy, qz): exact for float32
+// positions, minimum image when periodic
+MHIP_DEV void pair_offset(const ApplyArgs &a, const float (&c)[3], const float (&cl)[3], double qx, double qy,
+                          double qz, double (&d)[3]) {
+    d[0] = (double)c[0] - qx;
+    d[1] = (double)c[1] - qy;
+    d[2] = (double)c[2] - qz;
     if (a.p64) {
-        d[0] += (double)a.lx[p];
-        d[1] += (double)a.ly[p];
-        d[2] += (double)a.lz[p];
+        d[0] += (double)cl[0];
+        d[1] += (double)cl[1];
+        d[2] += (double)cl[2];
     }
     if (a.per) {
         const double box = (double)a.box;
 #pragma unroll
-        for (int c = 0; c < 3; c++) d[c] = wrap<double>(d[c], box);
+        for (int c2 = 0; c2 < 3; c2++) d[c2] = wrap<double>(d[c2], box);
     }
+}
+
+// neighbours per load batch of one lane (second-order fits): their indices
+// are loaded first, then their positions and field values, then the sums run.
+// A second-order lane serves all k neighbours of its query with few
+// wavefronts per SIMD (~230 registers), so a loop that waits on one
+// neighbour's dependent loads at a time stalls (measured 6.25 -> 5.24 ms at
+// 2^24, K = 2, with batches of 4; 7.2 ms with 8).  First-order fits keep one
+// neighbour per batch: their four lanes per query and higher occupancy hide
+// the latency already, and batching cost occupancy (2.61 -> 2.93 ms with 4).
+template <int ORDER>
+constexpr int apply_nb() { return ORDER == 1 ? 1 : 4; }
+
+// |d| from r2 = |d|^2 (float64): the hardware reciprocal square root and one
+// Newton step on the root give ~1e-16 relative error without the correctly
+// rounded expansion's scaling and fix-up instructions (apply_D1 2.27 ->
+// 2.05 ms at 2^24 with four lanes per query).  Tiny r2 (float64 inputs at
+// pathological scales, where the estimate would meet denormals) take sqrt.
+MHIP_DEV double pair_dist(double r2) {
+    if (r2 < 1e-280) return sqrt(r2);
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double r = r2 * y;
+    const double res = fma(-r, r, r2);
+    return fma(0.5 * y, res, r);
+}
+
+__global__ __launch_bounds__(256) void k_pack_xyzf(const float *__restrict__ sx, const float *__restrict__ sy,
+                                                  const float *__restrict__ sz, const float *__restrict__ f0,
+                                                  int64_t npad, float4 *__restrict__ out) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p < npad) out[p] = make_float4(sx[p], sy[p], sz[p], f0[p]);
 }
 
 // one query slot s of the list, fields [fb, fe): KC fields per pass over the
@@ -74,9 +109,14 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 
             for (int i = 0; i < ND; i++) B[f][i] = 0.0;
             fc[f] = f0 + f < a.K ? fs[(int64_t)(f0 + f) * a.npad + qp] : FT(0);
         }
-        auto pair = [&](uint32_t p, const double (&d)[3]) {
+        constexpr int APPLY_NB = apply_nb<ORDER>();
+        // field values of a batch are preloaded when they fit in registers
+        constexpr bool PRE = KC * APPLY_NB <= 32;
+        // packed (x, y, z, field 0): one 16-byte load per neighbour
+        const bool pack = std::is_same<FT, float>::value && a.xyzf != nullptr;
+        auto pair = [&](const double (&d)[3], const double (&df)[KC]) {
             double w = 1.0;
-            if (a.weighted) w = kernel_w<double>(sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
+            if (a.weighted) w = kernel_w<double>(pair_dist(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
             double av[ND];
             basis<DIM, ORDER, double>(d, av);
             double wa[ND];
@@ -90,17 +130,64 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 
 #pragma unroll
             for (int f = 0; f < KC; f++) {
                 if (f0 + f < a.K) {
-                    const double df = (double)fs[(int64_t)(f0 + f) * a.npad + p] - (double)fc[f];
 #pragma unroll
-                    for (int i = 0; i < ND; i++) B[f][i] = fma(wa[i], df, B[f][i]);
+                    for (int i = 0; i < ND; i++) B[f][i] = fma(wa[i], df[f], B[f][i]);
                 }
             }
         };
-        for (int j = gl; j < a.k; j += GW) {
-            const uint32_t p = a.nl_idx[(int64_t)j * a.nq + s];
-            double d[3];
-            pair_offset(a, p, qx, qy, qz, d);
-            pair(p, d);
+        for (int j0 = gl; j0 < a.k; j0 += GW * APPLY_NB) {
+            uint32_t pj[APPLY_NB];
+#pragma unroll
+            for (int u = 0; u < APPLY_NB; u++) {
+                const int j = j0 + u * GW;
+                pj[u] = j < a.k ? a.nl_idx[(int64_t)j * a.nq + s] : qp;
+            }
+            float cx[APPLY_NB][3], cl[APPLY_NB][3], pw[APPLY_NB];
+#pragma unroll
+            for (int u = 0; u < APPLY_NB; u++) {
+                if (pack) {
+                    const float4 v = a.xyzf[pj[u]];
+                    cx[u][0] = v.x;
+                    cx[u][1] = v.y;
+                    cx[u][2] = v.z;
+                    pw[u] = v.w;
+                } else {
+                    cx[u][0] = a.sx[pj[u]];
+                    cx[u][1] = a.sy[pj[u]];
+                    cx[u][2] = a.sz[pj[u]];
+                    pw[u] = 0.f;
+                }
+                cl[u][0] = a.p64 ? a.lx[pj[u]] : 0.f;
+                cl[u][1] = a.p64 ? a.ly[pj[u]] : 0.f;
+                cl[u][2] = a.p64 ? a.lz[pj[u]] : 0.f;
+            }
+            // field value f of neighbour u (field 0 from the pack when packed)
+            auto fval = [&](int u, int f) -> FT {
+                if (f0 + f >= a.K) return FT(0);
+                if (pack && f0 + f == 0) return (FT)pw[u];
+                return fs[(int64_t)(f0 + f) * a.npad + pj[u]];
+            };
+            FT fv[PRE ? APPLY_NB : 1][KC];
+            if (PRE) {
+#pragma unroll
+                for (int u = 0; u < APPLY_NB; u++) {
+#pragma unroll
+                    for (int f = 0; f < KC; f++) fv[u][f] = fval(u, f);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < APPLY_NB; u++) {
+                if (j0 + u * GW >= a.k) break;
+                double d[3];
+                pair_offset(a, cx[u], cl[u], qx, qy, qz, d);
+                double df[KC];
+#pragma unroll
+                for (int f = 0; f < KC; f++) {
+                    const FT v = PRE ? fv[PRE ? u : 0][f] : fval(u, f);
+                    df[f] = (double)v - (double)fc[f];
+                }
+                pair(d, df);
+            }
         }
         if (GW > 1) {
 #pragma unroll
@@ -140,11 +227,12 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 
     if (a.status && gl == 0) a.status[row] = a.or_status ? (a.status[row] | st) : st;
 }
 
-// lanes per query: 4 for first-order fits (more neighbour gathers in flight;
-// measured 2.79 vs 3.23 ms at 2^24, k = 32), 1 for the 9x9 second-order fits,
-// whose 63 float64 sums would cost more to combine than they gain (9.5 vs 7.0 ms)
+// lanes per query: 2 for first-order fits (measured at 2^24, k = 32, with the
+// packed (x, y, z, f) loads: 1.93 ms against 2.04 with 1 lane, 2.05 with 4
+// and 2.74 with 8), 1 for the 9x9 second-order fits, whose 63 float64 sums
+// would cost more to combine than they gain (9.5 vs 7.0 ms)
 template <int ORDER>
-constexpr int apply_gw() { return ORDER == 1 ? 4 : 1; }
+constexpr int apply_gw() { return ORDER == 1 ? 2 : 1; }
 
 template <int DIM, int ORDER, typename FT, int KC>
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
@@ -326,6 +414,7 @@ void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, i
     a->npad = ix->npad;
     a->k = nl->k;
     a->fs = fs;
+    a->xyzf = nullptr;
     a->K = K;
     a->weighted = 1;
     a->out1 = out1;
@@ -366,8 +455,14 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     if (K < 1) return fail(ctx, -1, "no field components");
     int rc = 0;
     const size_t fb = f_f64 ? 8 : 4;
-    void *fs = scratch(ctx, (size_t)K * ix->npad * fb, &rc);
-    if (!fs) return rc;
+    // float32 positions and fields: a packed (x, y, z, field 0) copy beside
+    // the sorted-space fields, so a neighbour is one 16-byte load
+    const bool pack = !f_f64 && !ix->pos_f64 && ix->dim == 3;
+    const size_t fs_bytes = ((size_t)K * ix->npad * fb + 255) & ~(size_t)255;
+    char *sc = (char *)scratch(ctx, fs_bytes + (pack ? (size_t)ix->npad * 16 : 0), &rc);
+    if (!sc) return rc;
+    void *fs = sc;
+    float4 *xyzf = pack ? (float4 *)(sc + fs_bytes) : nullptr;
     const dim3 gp((unsigned)((ix->npad + 255) / 256));
     ProfRegion prp(ctx, "apply_permute_fields");
     if (f_f64)
@@ -376,7 +471,11 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     else
         hipLaunchKernelGGL(k_permute_fields<float>, gp, dim3(256), 0, ctx->stream, (const float *)f,
                            nf_rows, K, ix->perm, ix->n, ix->npad, (float *)fs);
+    if (pack)
+        hipLaunchKernelGGL(k_pack_xyzf, gp, dim3(256), 0, ctx->stream, ix->sx, ix->sy, ix->sz, (const float *)fs,
+                           ix->npad, xyzf);
     ApplyArgs a;
+    a.xyzf = xyzf;
     a.sx = ix->sx;
     a.sy = ix->sy;
     a.sz = ix->sz;

@@ -93,6 +93,7 @@ struct ApplyArgs {
     int64_t nq, npad;
     int k;
     const void *fs;  // [K][npad] sorted-space fields
+    const float4 *xyzf;  // null, or [npad] (x, y, z, field 0) in sorted space (float32 positions and fields)
     int K;
     int weighted;
     float *out1, *out2;  // [row][K][DIM], [row][K][ND-DIM]
@@ -126,6 +127,13 @@ __global__ __launch_bounds__(256) void k_permute_fields_rows(const FT *__restric
     const uint32_t o = p < n ? perm[p] : 0xffffffffu;
     fs[e] = (o < nf_rows) ? f[(int64_t)o * K + c] : (FT)NAN;
 }
+
+// sorted-space pack (x, y, z, f) of float32 positions and the first field
+// component, so that a neighbour's position and value are one 16-byte load;
+// f0 is the sorted-space first field (k_permute_fields output)
+__global__ __launch_bounds__(256) void k_pack_xyzf(const float *__restrict__ sx, const float *__restrict__ sy,
+                                                  const float *__restrict__ sz, const float *__restrict__ f0,
+                                                  int64_t npad, float4 *__restrict__ out);
 
 // kernel-weighted neighbour statistics (mhip_kops.hip)
 int kernel_weights(mhip_ctx *ctx, const void *dist, int dist_f64, const void *h, int h_f64, int64_t nq, int k,

@@ -197,11 +197,9 @@ struct KnnArgs {
     const uint32_t *perm;
     int64_t nq, n;
     int k, dim;
-    const double *m_rows;  // per output row, null = 1
     const float *seed_h;   // Newton seed per output row, null = none
     const float *seed_r;   // search-radius bound per output row, null = none
     int redo;              // 1: second launch, redo waves with a radius-seed miss
-    float *h_out, *rho_out, *vol_out;
     int32_t *status_out;
     // per slot state
     float *tau;
@@ -273,17 +271,11 @@ MHIP_DEV double refine_h64(const KnnArgs &a, const QPos &q, double hd, F member)
     return (dndh > 0.0 && fabs(hn - hd) <= 1e-4 * hd) ? hn : hd;
 }
 
-// h, density and volume of one output row (Meshoid.py:165-170)
-MHIP_DEV void write_hsml(const KnnArgs &a, int64_t slot, uint32_t row, double h) {
+// smoothing length of one query slot (float32 copy and float64); the
+// row-order h, density and volume are formed from it by k_rows_out
+MHIP_DEV void write_h(const KnnArgs &a, int64_t slot, double h) {
     a.h_slot[slot] = (float)h;
     a.h64_slot[slot] = h;
-    const double vn = a.dim == 3 ? 4.18879020478639098 : (a.dim == 2 ? 3.14159265358979324 : 2.0);
-    const double m = a.m_rows ? a.m_rows[row] : 1.0;
-    const double hp = a.dim == 3 ? h * h * h : (a.dim == 2 ? h * h : h);
-    const double rho = (double)a.k * m / (vn * hp);
-    if (a.h_out) a.h_out[row] = (float)h;
-    if (a.rho_out) a.rho_out[row] = (float)rho;
-    if (a.vol_out) a.vol_out[row] = (float)(m / rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -790,7 +782,7 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
         }
     }
     if (live) {
-        write_hsml(a, slot, row, h64);
+        write_h(a, slot, h64);
         a.st_slot[slot] = st;
     }
     // ambiguous queries for the resolver, one vector atomic per wavefront by
@@ -899,10 +891,9 @@ MHIP_DEV void resolve_slot(const KnnArgs &a, const PT *__restrict__ pos_in, int6
     } else {
         st &= ~ST_COUNT;
         if (a.h_slot[slot] > 0.f && !(st & ST_HSML_NOCONV)) {
-            const uint32_t row = a.qrow ? a.qrow[slot] : a.perm[slot];
             const double h64 = refine_h64<PER, P64>(a, q, (double)a.h_slot[slot],
                                                     [&](int j) { return a.nl_idx[(int64_t)j * nq + slot]; });
-            write_hsml(a, slot, row, h64);
+            write_h(a, slot, h64);
         }
     }
     a.st_slot[slot] = st;
@@ -936,18 +927,37 @@ __global__ void k_qsetup_mask(const int64_t *__restrict__ qidx, int64_t nq,
 }
 
 __global__ void k_qsetup_finish(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-                                int64_t nq, uint32_t *__restrict__ qsorted, uint32_t *__restrict__ qrow) {
+                                int64_t nq, uint32_t *__restrict__ qsorted, uint32_t *__restrict__ qrow,
+                                uint32_t *__restrict__ rinv) {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= nq) return;
     qsorted[s] = (uint32_t)keys[s];
     qrow[s] = vals[s];
+    rinv[vals[s]] = (uint32_t)s;
 }
 
-__global__ void k_copy_status(const int32_t *__restrict__ st_slot, const uint32_t *__restrict__ qrow,
-                              const uint32_t *__restrict__ perm, int64_t nq, int32_t *__restrict__ out) {
-    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= nq) return;
-    out[qrow ? qrow[s] : perm[s]] = st_slot[s];
+// Row-order outputs from the slot-ordered results, one thread per output row
+// r (slot rslot[r]): h, density = k m / (V_d h^d), volume = m / density
+// (Meshoid.py:165-170) and the status bits.  The writes are coalesced; the
+// slot reads are gathers, which the slots of nearby rows share.
+__global__ __launch_bounds__(256) void k_rows_out(const uint32_t *__restrict__ rslot, int64_t nq, int k, int dim,
+                                                 const double *__restrict__ h64, const int32_t *__restrict__ st,
+                                                 const double *__restrict__ m_rows, float *__restrict__ h_out,
+                                                 float *__restrict__ rho_out, float *__restrict__ vol_out,
+                                                 int32_t *__restrict__ status_out) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= nq) return;
+    const uint32_t s = rslot[r];
+    if (status_out) status_out[r] = st[s];
+    if (!h_out && !rho_out && !vol_out) return;
+    const double h = h64[s];
+    const double vn = dim == 3 ? 4.18879020478639098 : (dim == 2 ? 3.14159265358979324 : 2.0);
+    const double m = m_rows ? m_rows[r] : 1.0;
+    const double hp = dim == 3 ? h * h * h : (dim == 2 ? h * h : h);
+    const double rho = (double)k * m / (vn * hp);
+    if (h_out) h_out[r] = (float)h;
+    if (rho_out) rho_out[r] = (float)rho;
+    if (vol_out) vol_out[r] = (float)(m / rho);
 }
 
 static int nl_grow(mhip_ctx *ctx, void **p, size_t bytes) {
@@ -975,7 +985,7 @@ static void launch_main(const KnnArgs &a, bool per, bool p64, dim3 g, hipStream_
 static int nl_reserve(mhip_ctx *ctx, mhip_nlist *nl, int64_t nq, int k) {
     int rc = 0;
     if (nl->cap < nq || nl->k < k) {
-        void **arr[] = {(void **)&nl->qsorted, (void **)&nl->qrow, (void **)&nl->tau,
+        void **arr[] = {(void **)&nl->qsorted, (void **)&nl->qrow, (void **)&nl->rinv, (void **)&nl->tau,
                         (void **)&nl->nlt,     (void **)&nl->h,    (void **)&nl->status};
         for (auto p : arr)
             if ((rc = nl_grow(ctx, p, nq * 4))) return rc;
@@ -1007,7 +1017,7 @@ static int query_setup(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx,
     uint32_t *vs;
     if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nq, bits, &ks, &vs))) return rc;
     hipLaunchKernelGGL(k_qsetup_finish, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream, ks, vs,
-                       nq, nl->qsorted, nl->qrow);
+                       nq, nl->qsorted, nl->qrow, nl->rinv);
     return 0;
 }
 
@@ -1060,13 +1070,9 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     a.n = ix->n;
     a.k = k;
     a.dim = ix->dim;
-    a.m_rows = m_rows;
     a.seed_h = seed_h;
     a.seed_r = seed_r;
     a.redo = 0;
-    a.h_out = h_out;
-    a.rho_out = rho_out;
-    a.vol_out = vol_out;
     a.status_out = status_out;
     a.tau = nl->tau;
     a.nlt = nl->nlt;
@@ -1140,9 +1146,12 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
                                (const float *)ix->pos_in);
     }
     pr3.end();
-    if (status_out)
-        hipLaunchKernelGGL(k_copy_status, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s,
-                           nl->status, a.qrow, ix->perm, nq, status_out);
+    if (status_out || h_out || rho_out || vol_out) {
+        ProfRegion pr4(ctx, "knn_rows_out");
+        hipLaunchKernelGGL(k_rows_out, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s,
+                           masked ? nl->rinv : ix->iperm, nq, k, ix->dim, nl->h64, nl->status, m_rows, h_out,
+                           rho_out, vol_out, status_out);
+    }
     return check_hip(ctx, hipGetLastError(), "knn launch");
 }
 
