@@ -160,7 +160,7 @@ def batched_form(args, torch, dev, log):
     f_b = [torch.from_numpy(f).to(dev) for _, f, _ in snaps]
     host = snaps[1] if ns > 1 else snaps[0]
     del snaps
-    B = SnapshotBatch(sizes, ks, 32, nbranch=8)
+    B = SnapshotBatch(sizes, ks, 32, nbranch=4)
     log("batched form: first eager run (field-tile autotune)")
     B.run(pos_b, f_b, graph=False)
 

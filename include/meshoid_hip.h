@@ -336,13 +336,15 @@ int mhip_reconstruct(mhip_ctx *ctx, const double *points, int64_t nt, int dim, d
  * float32 fields (n[s], K); per snapshot: h, rho float32[n[s]], the weighted
  * gradient of every field grad float32[n[s] * K * 3] (as mhip_apply_D, order
  * 1) and status int32[n[s]] (kNN bits OR derivative bits).  Snapshots are
- * spread over nbranch streams; with use_graph the whole batch (fork/join of
- * the branches + one contraction launch over all snapshots) is captured once
- * into a HIP graph and replayed (recaptured when the buffers change).  The
- * first run is eager and times the contraction's candidates (K >= 8: a group of
- * 16/32/64 threads per query over row-major fields, wide = 1; fewer fields:
- * 2/4/8/16 fields per thread); mhip_batch_info reports them and the choice.
- * Non-periodic sets only.
+ * spread over up to nbranch streams (at most 4); with use_graph the whole
+ * batch (fork/join of the branches + one contraction launch over all
+ * snapshots) is captured once into a HIP graph and replayed (recaptured when
+ * the buffers change).  The first run is eager and times the number of
+ * branches in use (1, 2 or 4, at most nbranch) and then the contraction's
+ * candidates (K >= 8: a group of 16/32/64 threads per query over row-major
+ * fields, wide = 1, plus, for K = 16, 32 or 64, the matrix-core kernel;
+ * fewer fields: 2/4/8/16 fields per thread); mhip_batch_info and
+ * mhip_batch_tuning report them and the choices.  Non-periodic sets only.
  */
 int mhip_batch_create(mhip_ctx *ctx, int nsnap, const int64_t *n, const int *k, int K, int nbranch,
                       mhip_batch **out);
@@ -351,6 +353,16 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
                    int use_graph);
 int mhip_batch_info(const mhip_batch *b, int *wide, int *choice, int *candidates4, float *tune_ms4,
                     int *nbranch);
+/* The first run's timings: ncand candidates (<= 4) with kind4 (0 tile, 1
+ * wide, 2 matrix-core), param4 (fields per thread or threads per query) and
+ * ms4; choice = index of the candidate in use; nb_use = branches in use;
+ * branch_ms3 = the snapshots' work with 1, 2 and 4 branches (0: not timed).
+ * Any output may be NULL. */
+int mhip_batch_tuning(const mhip_batch *b, int *ncand, int *kind4, int *param4, float *ms4, int *choice,
+                      int *nb_use, float *branch_ms3);
+/* Test hook, before the first run: use candidate cand_index (-1: timed) and
+ * nbranch branches (0: timed; else 1, 2 or 4, at most the batch's streams). */
+int mhip_batch_force(mhip_batch *b, int cand_index, int nbranch);
 int mhip_batch_destroy(mhip_batch *b);
 
 /*

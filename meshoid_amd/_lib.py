@@ -31,6 +31,7 @@ EXPORTED = [
     "mhip_seed_predict", "mhip_mlp_train_step", "mhip_mlp_pack", "mhip_mlp_blob_bytes",
     "mhip_kernel_weights", "mhip_kernel_average", "mhip_kernel_variance", "mhip_integrate",
     "mhip_batch_create", "mhip_batch_run", "mhip_batch_info", "mhip_batch_destroy",
+    "mhip_batch_tuning", "mhip_batch_force",
     "mhip_index_order", "mhip_nearest", "mhip_unique_inverse", "mhip_reconstruct", "mhip_nlist_subset",
     "mhip_surface_density", "mhip_grid_density", "mhip_radtransfer", "mhip_nlist_export_f64",
     "mhip_nlist_hsml", "mhip_integrate_noalloc",
@@ -87,6 +88,8 @@ _SIGS = {
     "mhip_batch_run": ([_p, _p, _p, _p, _p, _p, _p, _p, _i], _i),
     "mhip_batch_info": ([_p, _p, _p, _p, _p, _p], _i),
     "mhip_batch_destroy": ([_p], _i),
+    "mhip_batch_tuning": ([_p, _p, _p, _p, _p, _p, _p, _p], _i),
+    "mhip_batch_force": ([_p, _i, _i], _i),
     "mhip_index_order": ([_p, _p, _p], _i),
     "mhip_nearest": ([_p, _p, _p, _i64, _i, _p], _i),
     "mhip_nlist_subset": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _p], _i),

@@ -27,9 +27,10 @@ def _ptrs(ts):
 
 class SnapshotBatch:
     """sizes: particles per snapshot; des_ngb: int or one per snapshot;
-    K: field components per particle; nbranch: concurrent streams."""
+    K: field components per particle; nbranch: most concurrent streams (the
+    first run times 1, 2 and 4 of them, up to nbranch)."""
 
-    def __init__(self, sizes, des_ngb, K, nbranch=8, device=None):
+    def __init__(self, sizes, des_ngb, K, nbranch=4, device=None):
         self.sizes = [int(n) for n in sizes]
         ns = len(self.sizes)
         if isinstance(des_ngb, int):
@@ -87,17 +88,29 @@ class SnapshotBatch:
         cur.wait_stream(self._stream)
         return o
 
+    def force(self, candidate=-1, nbranch=0):
+        """Test hook, before the first run: use candidate index ``candidate``
+        of the tuning list (-1: timed) and ``nbranch`` streams (0: timed)."""
+        raise_for(lib.mhip_batch_force(self.h, int(candidate), int(nbranch)))
+
     def info(self):
-        """The contraction kernel chosen by the first run's timing: kernel
-        "wide" (threads per query) or "tile" (fields per thread), the chosen
-        value and the measured ms of every candidate."""
-        wide, choice, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        cand = (ctypes.c_int * 4)()
+        """The first run's timings and choices: every contraction candidate
+        ("tile": fields per thread, "wide": threads per query on the VALU,
+        "mfma": the same on the matrix cores) with its ms, the candidate in
+        use, and the branch streams in use with the timings of 1, 2 and 4."""
+        nc, choice, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        kind = (ctypes.c_int * 4)()
+        param = (ctypes.c_int * 4)()
         ms = (ctypes.c_float * 4)()
-        raise_for(lib.mhip_batch_info(self.h, ctypes.byref(wide), ctypes.byref(choice), cand, ms,
-                                      ctypes.byref(nb)))
-        return {"kernel": "wide" if wide.value else "tile", "choice": choice.value, "branches": nb.value,
-                "candidates_ms": {int(cand[i]): round(float(ms[i]), 4) for i in range(4) if cand[i]}}
+        bms = (ctypes.c_float * 3)()
+        raise_for(lib.mhip_batch_tuning(self.h, ctypes.byref(nc), kind, param, ms, ctypes.byref(choice),
+                                        ctypes.byref(nb), bms))
+        names = {0: "tile", 1: "wide", 2: "mfma"}
+        cands = [{"kernel": names[kind[i]], "param": int(param[i]), "ms": round(float(ms[i]), 4)}
+                 for i in range(nc.value)]
+        ch = cands[choice.value] if 0 <= choice.value < nc.value else None
+        return {"candidates": cands, "choice": ch, "branches": nb.value,
+                "branch_ms": {n: round(float(bms[i]), 3) for i, n in enumerate((1, 2, 4)) if bms[i] > 0}}
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -18,9 +18,13 @@
 // bbox / NaN records are validated after each run.
 //
 // The contraction kernel: with K >= 8 fields, k_apply_wide (a thread group
-// per query, row-major fields; group width 16, 32 or 64 threads); with fewer,
-// the tile kernel (2, 4, 8 or 16 fields per thread).  The variant is chosen by
-// timing every candidate on the first eager run.
+// per query, row-major fields; group width 16, 32 or 64 threads) and, when K
+// is 16, 32 or 64, k_apply_mfma (the same with the field contraction on the
+// matrix cores); with fewer, the tile kernel (2, 4, 8 or 16 fields per
+// thread).  The first eager run times the number of branch streams in use
+// (1, 2 or 4, at most the nbranch given: the device runs 4 hardware queues
+// per process) and then every contraction candidate, and keeps the fastest of
+// each.
 #include <algorithm>
 #include <new>
 #include <numeric>
@@ -30,7 +34,10 @@
 #include "mhip_internal.hpp"
 
 struct mhip_batch {
-    int nsnap = 0, K = 0, nbranch = 0;
+    int nsnap = 0, K = 0, nbranch = 0;  // nbranch: streams created
+    int nb_use = 0;                     // branches in use (timed on the first run)
+    int force_cand = -1, force_nb = 0;  // test hook: fixed choices (-1 / 0: timed)
+    float branch_ms[3] = {0, 0, 0};     // per branch count 1, 2, 4 (first run)
     std::vector<int64_t> n;
     std::vector<int> k, branch_of;
     std::vector<mhip_index *> ix;
@@ -43,7 +50,7 @@ struct mhip_batch {
     float *fsorted = nullptr;           // per snapshot [K][npad] sorted-space fields
     std::vector<int64_t> foff;          // offsets into fsorted (floats)
     mhip::ApplyArgs *tab_d = nullptr;   // [nsnap]
-    int wide = 0;                       // contraction kernel family (1: k_apply_wide, row-major fields)
+    int wide = 0;                       // contraction kernel family (1: row-major fields)
     int64_t *blk0_d = nullptr;          // per candidate [nsnap + 1] block prefixes
     int64_t nblocks[4] = {0, 0, 0, 0};  // per candidate
     int choice = -1;                    // chosen candidate
@@ -57,19 +64,43 @@ struct mhip_batch {
 namespace mhip {
 
 // contraction candidates: fields per thread for the tile kernel (256
-// queries per block), threads per query for the wide kernel
+// queries per block), threads per query for the wide and matrix-core kernels
+enum CandKind { CAND_TILE = 0, CAND_WIDE = 1, CAND_MFMA = 2 };
 struct Cand {
-    int param, qpb;
+    int kind, param, qpb;
 };
-static const Cand TILE_CAND[4] = {{2, 256}, {4, 256}, {8, 256}, {16, 256}};
-static const Cand WIDE_CAND[4] = {{16, 16}, {32, 8}, {64, 4}, {0, 0}};
-static int ncand(const mhip_batch *b) { return b->wide ? 3 : 4; }
-static Cand cand(const mhip_batch *b, int c) { return b->wide ? WIDE_CAND[c] : TILE_CAND[c]; }
+static const Cand TILE_CAND[4] = {{CAND_TILE, 2, 256}, {CAND_TILE, 4, 256}, {CAND_TILE, 8, 256}, {CAND_TILE, 16, 256}};
+static const Cand WIDE_CAND[3] = {{CAND_WIDE, 16, 16}, {CAND_WIDE, 32, 8}, {CAND_WIDE, 64, 4}};
+static bool has_mfma(const mhip_batch *b) { return b->wide && (b->K == 16 || b->K == 32 || b->K == 64); }
+static int ncand(const mhip_batch *b) { return b->wide ? (has_mfma(b) ? 4 : 3) : 4; }
+static Cand cand(const mhip_batch *b, int c) {
+    if (!b->wide) return TILE_CAND[c];
+    if (c < 3) return WIDE_CAND[c];
+    return Cand{CAND_MFMA, b->K, 256 / b->K};  // K == threads per query
+}
 
 static int launch_contraction(mhip_ctx *ctx, mhip_batch *b, int c) {
     const int64_t *blk0 = b->blk0_d + (int64_t)c * (b->nsnap + 1);
-    if (b->wide) return apply_wide_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cand(b, c).param);
-    return apply_batch_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cand(b, c).param, b->K);
+    const Cand cd = cand(b, c);
+    if (cd.kind == CAND_MFMA) return apply_mfma_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cd.param);
+    if (cd.kind == CAND_WIDE) return apply_wide_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cd.param);
+    return apply_batch_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cd.param, b->K);
+}
+
+// snapshots to the first nb branches: largest first, each to the least
+// loaded branch
+static void assign_branches(mhip_batch *b, int nb) {
+    std::vector<int> order(b->nsnap);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->n[x] > b->n[y]; });
+    std::vector<double> load(nb, 0.0);
+    b->branch_of.assign(b->nsnap, 0);
+    for (int s : order) {
+        const int i = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->branch_of[s] = i;
+        load[i] += (double)b->n[s] * b->k[s];
+    }
+    b->nb_use = nb;
 }
 
 static int64_t npad_of(int64_t n) { return (n + LEAF - 1) / LEAF * LEAF; }
@@ -99,7 +130,7 @@ static int enqueue_snapshots(mhip_ctx *ctx, mhip_batch *b, const float *const *p
                              float *const *h, float *const *rho, int32_t *const *status) {
     int rc = check_hip(ctx, hipEventRecord(b->fork, ctx->stream), "batch fork");
     if (rc) return rc;
-    for (int i = 0; i < b->nbranch; i++)
+    for (int i = 0; i < b->nb_use; i++)
         if ((rc = check_hip(ctx, hipStreamWaitEvent(b->streams[i], b->fork, 0), "batch fork wait"))) return rc;
     for (int s = 0; s < b->nsnap; s++) {
         if ((rc = enqueue_snapshot(b, s, pos[s], f[s], h[s], rho[s], status[s]))) {
@@ -107,7 +138,7 @@ static int enqueue_snapshots(mhip_ctx *ctx, mhip_batch *b, const float *const *p
             return rc;
         }
     }
-    for (int i = 0; i < b->nbranch; i++) {
+    for (int i = 0; i < b->nb_use; i++) {
         if ((rc = check_hip(ctx, hipEventRecord(b->join[i], b->streams[i]), "batch join"))) return rc;
         if ((rc = check_hip(ctx, hipStreamWaitEvent(ctx->stream, b->join[i], 0), "batch join wait"))) return rc;
     }
@@ -166,7 +197,7 @@ int mhip_batch_create(mhip_ctx *ctx, int nsnap, const int64_t *n, const int *k, 
     if (nsnap < 1 || !n || !k) return fail(ctx, MHIP_ERR_VALUE, "batch needs at least one snapshot");
     if (K < 1) return fail(ctx, MHIP_ERR_VALUE, "no field components");
     if (nbranch < 1) nbranch = 1;
-    nbranch = std::min(nbranch, nsnap);
+    nbranch = std::min(std::min(nbranch, nsnap), 4);
     for (int s = 0; s < nsnap; s++) {
         if (k[s] < 2 || k[s] > 128) return fail(ctx, MHIP_ERR_UNSUPPORTED, "des_ngb must be in [2, 128]");
         if (n[s] < k[s]) return fail(ctx, MHIP_ERR_VALUE, "des_ngb exceeds the number of particles");
@@ -180,17 +211,7 @@ int mhip_batch_create(mhip_ctx *ctx, int nsnap, const int64_t *n, const int *k, 
     b->wide = K >= 8;  // many fields: one query per thread group, coalesced field rows
     b->n.assign(n, n + nsnap);
     b->k.assign(k, k + nsnap);
-    // largest snapshots first, each to the least loaded branch
-    std::vector<int> order(nsnap);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
-    std::vector<double> load(nbranch, 0.0);
-    b->branch_of.assign(nsnap, 0);
-    for (int s : order) {
-        const int i = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        b->branch_of[s] = i;
-        load[i] += (double)n[s] * k[s];
-    }
+    assign_branches(b, nbranch);
     int rc = 0;
     auto bail = [&](int code) {
         mhip_batch_destroy(b);
@@ -254,6 +275,37 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
         // tile.  The argument table holds the lists' device pointers, which
         // exist once the snapshots are enqueued.
         if ((rc = enqueue_snapshots(ctx, b, pos, f, h, rho, status))) return rc;
+        if (b->choice < 0) {
+            // branch count: the snapshots' work timed with 1, 2 and 4
+            // streams in use (the buffers exist after the run above)
+            if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch warm-up"))) return rc;
+            int best_nb = b->nbranch;
+            float best_t = 0.f;
+            const int nbs[3] = {1, 2, 4};
+            for (int t = 0; t < 3; t++) {
+                b->branch_ms[t] = 0.f;
+                if (nbs[t] > b->nbranch || (b->force_nb && nbs[t] != b->force_nb)) continue;
+                assign_branches(b, nbs[t]);
+                hipEvent_t t0, t1;
+                (void)hipEventCreate(&t0);
+                (void)hipEventCreate(&t1);
+                (void)hipEventRecord(t0, ctx->stream);
+                rc = enqueue_snapshots(ctx, b, pos, f, h, rho, status);
+                (void)hipEventRecord(t1, ctx->stream);
+                if (!rc) rc = check_hip(ctx, hipEventSynchronize(t1), "batch branch timing");
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, t0, t1);
+                (void)hipEventDestroy(t0);
+                (void)hipEventDestroy(t1);
+                if (rc) return rc;
+                b->branch_ms[t] = ms;
+                if (best_t == 0.f || ms < best_t) {
+                    best_t = ms;
+                    best_nb = nbs[t];
+                }
+            }
+            assign_branches(b, best_nb);
+        }
         if ((rc = upload_table(ctx, b, grad, status))) return rc;
         if (b->choice < 0) {
             if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch warm-up"))) return rc;
@@ -262,6 +314,7 @@ int mhip_batch_run(mhip_ctx *ctx, mhip_batch *b, const float *const *pos, const 
             (void)hipEventCreate(&e1);
             float best = 0.f;
             for (int c = 0; c < ncand(b); c++) {
+                if (b->force_cand >= 0 && c != b->force_cand) continue;
                 float ms = 0.f;
                 for (int rep = 0; rep < 3; rep++) {
                     (void)hipEventRecord(e0, ctx->stream);
@@ -326,7 +379,36 @@ int mhip_batch_info(const mhip_batch *b, int *wide, int *choice, int *cand4, flo
         if (cand4) cand4[c] = in ? cand(b, c).param : 0;
         if (tune_ms4) tune_ms4[c] = in ? b->tune_ms[c] : 0.f;
     }
-    if (nbranch) *nbranch = b->nbranch;
+    if (nbranch) *nbranch = b->nb_use;
+    return MHIP_OK;
+}
+
+int mhip_batch_tuning(const mhip_batch *b, int *ncand_out, int *kind4, int *param4, float *ms4, int *choice,
+                      int *nb_use, float *branch_ms3) {
+    if (!b) return MHIP_ERR_VALUE;
+    const int nc = ncand(b);
+    if (ncand_out) *ncand_out = nc;
+    for (int c = 0; c < 4; c++) {
+        const bool in = c < nc;
+        if (kind4) kind4[c] = in ? cand(b, c).kind : -1;
+        if (param4) param4[c] = in ? cand(b, c).param : 0;
+        if (ms4) ms4[c] = in ? b->tune_ms[c] : 0.f;
+    }
+    if (choice) *choice = b->choice;
+    if (nb_use) *nb_use = b->nb_use;
+    for (int t = 0; t < 3; t++)
+        if (branch_ms3) branch_ms3[t] = b->branch_ms[t];
+    return MHIP_OK;
+}
+
+int mhip_batch_force(mhip_batch *b, int cand_index, int nbranch) {
+    if (!b) return MHIP_ERR_VALUE;
+    if (b->warmed) return MHIP_ERR_VALUE;  // choices are made on the first run
+    if (cand_index >= ncand(b)) return MHIP_ERR_VALUE;
+    if (nbranch != 0 && nbranch != 1 && nbranch != 2 && nbranch != 4) return MHIP_ERR_VALUE;
+    if (nbranch > b->nbranch) return MHIP_ERR_VALUE;
+    b->force_cand = cand_index;
+    b->force_nb = nbranch;
     return MHIP_OK;
 }
 

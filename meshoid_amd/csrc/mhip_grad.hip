@@ -285,11 +285,19 @@ __global__ __launch_bounds__(256) void k_apply_batch(const ApplyArgs *__restrict
 constexpr int WIDE_KMAX = 128;
 
 template <int KW>
-__global__ __launch_bounds__(256) void k_apply_wide(const ApplyArgs *__restrict__ tab,
-                                                    const int64_t *__restrict__ blk0, int nsnap) {
+struct WideQuery {
+    int64_t s, sc;
+    bool live;
+    uint32_t qp, row;
+    int32_t st;
+    bool ok;
+};
+
+// the block's snapshot (binary search of the block prefix) and phase 1
+template <int KW>
+MHIP_DEV WideQuery<KW> wide_phase1(const ApplyArgs *__restrict__ tab, const int64_t *__restrict__ blk0, int nsnap,
+                                   double (*s_g)[WIDE_KMAX][3], uint32_t (*s_p)[WIDE_KMAX], const ApplyArgs *&ap) {
     constexpr int G = 256 / KW;
-    __shared__ double s_g[G][WIDE_KMAX][3];
-    __shared__ uint32_t s_p[G][WIDE_KMAX];
     const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
     int lo = 0, hi = nsnap;
     while (hi - lo > 1) {
@@ -297,26 +305,27 @@ __global__ __launch_bounds__(256) void k_apply_wide(const ApplyArgs *__restrict_
         if (blk0[mid] <= b) lo = mid;
         else hi = mid;
     }
-    const ApplyArgs a = tab[lo];
+    ap = tab + lo;
+    const ApplyArgs &a = *ap;
     const int g = threadIdx.x / KW, f = threadIdx.x % KW;
-    const int64_t s = (b - blk0[lo]) * G + g;
-    const bool live = s < a.nq;
-    const int64_t sc = live ? s : a.nq - 1;  // padding queries repeat the last one
-    const uint32_t qp = a.qsorted ? a.qsorted[sc] : (uint32_t)sc;
-    const uint32_t row = a.qrow ? a.qrow[sc] : a.perm[sc];
-    const double qx = (double)a.sx[qp], qy = (double)a.sy[qp], qz = (double)a.sz[qp];
-    const double hd = a.h[sc];
+    WideQuery<KW> w;
+    w.s = (b - blk0[lo]) * G + g;
+    w.live = w.s < a.nq;
+    w.sc = w.live ? w.s : a.nq - 1;  // padding queries repeat the last one
+    w.qp = a.qsorted ? a.qsorted[w.sc] : (uint32_t)w.sc;
+    w.row = a.qrow ? a.qrow[w.sc] : a.perm[w.sc];
+    const double qx = (double)a.sx[w.qp], qy = (double)a.sy[w.qp], qz = (double)a.sz[w.qp];
+    const double hd = a.h[w.sc];
     const double ih = hd > 0.0 ? 1.0 / hd : (double)INFINITY;
     const int k = a.k;
-    // phase 1: w_j a_j and the moment matrix
     double E[6] = {0, 0, 0, 0, 0, 0};
     for (int j = f; j < k; j += KW) {
-        const uint32_t p = a.nl_idx[(int64_t)j * a.nq + sc];
+        const uint32_t p = a.nl_idx[(int64_t)j * a.nq + w.sc];
         const double d[3] = {(double)a.sx[p] - qx, (double)a.sy[p] - qy, (double)a.sz[p] - qz};
-        const double w = kernel_w<double>(sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
+        const double wt = kernel_w<double>(sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
         double wa[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) wa[i] = w * d[i];
+        for (int i = 0; i < 3; i++) wa[i] = wt * d[i];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
 #pragma unroll
@@ -331,10 +340,10 @@ __global__ __launch_bounds__(256) void k_apply_wide(const ApplyArgs *__restrict_
 #pragma unroll
         for (int e = 0; e < 6; e++) E[e] += __shfl_xor(E[e], o);
     }
-    int32_t st = 0;
-    if (zero_diagonal<3>(E)) st |= ST_SINGULAR;
-    const bool ok = ldlt<3>(E);
-    if (!ok) st |= ST_RANKDEF;
+    w.st = 0;
+    if (zero_diagonal<3>(E)) w.st |= ST_SINGULAR;
+    w.ok = ldlt<3>(E);
+    if (!w.ok) w.st |= ST_RANKDEF;
     __syncthreads();
     for (int j = f; j < k; j += KW) {
         double v[3] = {s_g[g][j][0], s_g[g][j][1], s_g[g][j][2]};
@@ -343,26 +352,118 @@ __global__ __launch_bounds__(256) void k_apply_wide(const ApplyArgs *__restrict_
         for (int i = 0; i < 3; i++) s_g[g][j][i] = v[i];
     }
     __syncthreads();
-    // phase 2: fields f, f + KW, ...
+    return w;
+}
+
+template <int KW>
+__global__ __launch_bounds__(256) void k_apply_wide(const ApplyArgs *__restrict__ tab,
+                                                    const int64_t *__restrict__ blk0, int nsnap) {
+    constexpr int G = 256 / KW;
+    __shared__ double s_g[G][WIDE_KMAX][3];
+    __shared__ uint32_t s_p[G][WIDE_KMAX];
+    const ApplyArgs *ap;
+    const WideQuery<KW> w = wide_phase1<KW>(tab, blk0, nsnap, s_g, s_p, ap);
+    const ApplyArgs &a = *ap;
+    const int g = threadIdx.x / KW, f = threadIdx.x % KW;
+    const int k = a.k;
+    // phase 2: fields f, f + KW, ...; the neighbour loop is unrolled so that
+    // several neighbours' field loads are in flight at once
     const float *fs = (const float *)a.fs;
     for (int c = f; c < a.K; c += KW) {
-        const double fi = (double)fs[(int64_t)qp * a.K + c];
+        const double fi = (double)fs[(int64_t)w.qp * a.K + c];
         double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+#pragma unroll 8
         for (int j = 0; j < k; j++) {
             const double df = (double)fs[(int64_t)s_p[g][j] * a.K + c] - fi;
             o0 = fma(s_g[g][j][0], df, o0);
             o1 = fma(s_g[g][j][1], df, o1);
             o2 = fma(s_g[g][j][2], df, o2);
         }
-        if (!ok) o0 = o1 = o2 = (double)NAN;
-        if (live) {
-            float *o = a.out1 + ((int64_t)row * a.K + c) * 3;
+        if (!w.ok) o0 = o1 = o2 = (double)NAN;
+        if (w.live) {
+            float *o = a.out1 + ((int64_t)w.row * a.K + c) * 3;
             o[0] = (float)o0;
             o[1] = (float)o1;
             o[2] = (float)o2;
         }
     }
-    if (live && f == 0 && a.status) a.status[row] = a.or_status ? (a.status[row] | st) : st;
+    if (w.live && f == 0 && a.status) a.status[w.row] = a.or_status ? (a.status[w.row] | w.st) : w.st;
+}
+
+// The same operator with phase 2 on the matrix cores (K == KW fields, KW in
+// {16, 32, 64}): v_mfma_f32_4x4x1_16b_f32 computes 16 independent 4 x 4
+// blocks C_b += A_b B_b with A_b 4 x 1 and B_b 1 x 4.  Block b of a wavefront
+// is (query q = b / (KW / 4) of the wavefront's 64 / KW queries, field group
+// fg = b % (KW / 4)); its rows are the gradient components (g_j, row 3 zero)
+// and its columns 4 fields (f_j - f_i), one neighbour j per instruction:
+// C_b[c][t] = sum_j g_j[c] (f_j - f_i)[4 fg + t].  Lane l supplies A_b[l % 4]
+// and B_b[l % 4] of block l / 4 and holds column l % 4 of C_b (rows in the 4
+// result registers).  float32 products, float32 accumulation in four
+// partial sums (neighbours j mod 4), added at the end.
+template <int KW>
+__global__ __launch_bounds__(256) void k_apply_mfma(const ApplyArgs *__restrict__ tab,
+                                                    const int64_t *__restrict__ blk0, int nsnap) {
+    constexpr int G = 256 / KW;
+    constexpr int NGRP = KW / 4;  // field groups per query
+    __shared__ double s_g[G][WIDE_KMAX][3];
+    __shared__ uint32_t s_p[G][WIDE_KMAX];
+    const ApplyArgs *ap;
+    const WideQuery<KW> w = wide_phase1<KW>(tab, blk0, nsnap, s_g, s_p, ap);
+    const ApplyArgs &a = *ap;
+    const int g = threadIdx.x / KW, f = threadIdx.x % KW;
+    // this lane's block: its query is g (a wavefront holds 64 / KW whole
+    // queries, as in phase 1), its field group and column
+    const int wl = threadIdx.x & 63;
+    const int blk = wl >> 2, t = wl & 3;
+    const int gq = (threadIdx.x >> 6) * (64 / KW) + blk / NGRP;  // == g
+    const int field = 4 * (blk % NGRP) + t;
+    const int k = a.k;
+    const float *fs = (const float *)a.fs;
+    // the query of this lane's block is the lane's own query (g == gq), so its
+    // own position and row come from phase 1
+    const float fi = fs[(int64_t)w.qp * a.K + field];
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    // four accumulators (neighbours j mod 4): consecutive matrix instructions
+    // do not wait on each other's results
+    v4f acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[u] = v4f{0.f, 0.f, 0.f, 0.f};
+    int j = 0;
+    for (; j + 4 <= k; j += 4) {
+        float av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            av[u] = t < 3 ? (float)s_g[gq][j + u][t] : 0.f;
+            bv[u] = fs[(int64_t)s_p[gq][j + u] * a.K + field] - fi;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc[u] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[u], bv[u], acc[u], 0, 0, 0);
+    }
+    for (; j < k; j++) {
+        const float av = t < 3 ? (float)s_g[gq][j][t] : 0.f;
+        const float bv = fs[(int64_t)s_p[gq][j] * a.K + field] - fi;
+        acc[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(av, bv, acc[0], 0, 0, 0);
+    }
+    const v4f res = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    if (w.live) {
+        float *o = a.out1 + ((int64_t)w.row * a.K + field) * 3;
+        o[0] = w.ok ? res[0] : NAN;
+        o[1] = w.ok ? res[1] : NAN;
+        o[2] = w.ok ? res[2] : NAN;
+    }
+    if (w.live && f == 0 && a.status) a.status[w.row] = a.or_status ? (a.status[w.row] | w.st) : w.st;
+}
+
+int apply_mfma_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
+                      int64_t nblocks, int kw) {
+    const dim3 g((unsigned)nblocks), b(256);
+    switch (kw) {
+        case 16: hipLaunchKernelGGL(k_apply_mfma<16>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        case 32: hipLaunchKernelGGL(k_apply_mfma<32>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        case 64: hipLaunchKernelGGL(k_apply_mfma<64>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
+        default: return fail(ctx, -1, "unsupported field group");
+    }
+    return check_hip(ctx, hipGetLastError(), "mfma apply launch");
 }
 
 int apply_wide_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,

@@ -113,6 +113,9 @@ void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, i
 // holds 256 / kw queries; blk0 counts query blocks of that size
 int apply_wide_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
                       int64_t nblocks, int kw);
+// the same with the field contraction on the matrix cores (K == kw fields)
+int apply_mfma_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
+                      int64_t nblocks, int kw);
 
 // field rows (original order, K components each) -> sorted-space rows
 // [npad][K]; rows beyond nf_rows and the padding read NaN

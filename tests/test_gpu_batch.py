@@ -102,7 +102,8 @@ def test_batch_matches_single_path():
     o = B.run(pos, fld, graph=False)
     info = B.info()
     say("batch info", info)
-    assert info["choice"] in info["candidates_ms"]
+    assert info["choice"] in info["candidates"] and info["choice"]["kernel"] == "tile"
+    assert 1 <= info["branches"] <= 3
     for s in range(len(sizes)):
         h, rho, d1, st, cond = _single(pos[s], fld[s], ks[s])
         assert same(o["h"][s], h), s
@@ -126,7 +127,8 @@ def test_batch_wide_contraction():
     o = B.run(pos, fld, graph=False)
     info = B.info()
     say("batch info", info)
-    assert info["kernel"] == "wide" and info["choice"] in (16, 32, 64)
+    assert info["choice"]["kernel"] == "wide" and info["choice"]["param"] in (16, 32, 64)
+    assert [c["kernel"] for c in info["candidates"]] == ["wide"] * 3  # K = 12: no matrix-core kernel
     for s in range(len(sizes)):
         h, rho, d1, st, cond = _single(pos[s], fld[s], ks[s])
         assert same(o["h"][s], h) and same(o["rho"][s], rho) and same(o["status"][s], st), s
@@ -179,3 +181,74 @@ def test_batch_rejects_bad_input():
     B = SnapshotBatch([3000, 2000], 32, 2, nbranch=2)
     with pytest.raises(ValueError):
         B.run([pos[0], pos[1][:1000]], fld, graph=False)  # shape differs from the layout
+
+
+@pytest.mark.parametrize("K", [16, 32])
+def test_batch_mfma_contraction(K):
+    """The matrix-core contraction (forced, so it is checked whatever the
+    timing picks) against the single-set float64 path.  It multiplies and
+    sums in float32, so each output component may differ by the float32
+    rounding of its k terms: |err| <= (k + 4) 2^-24 sum_j |g_j| |f_j - f_i|,
+    with g_j the oracle's weights on the same neighbours (a wrong lane layout
+    gives errors of the order of the result itself)."""
+    torch = _torch()
+    from meshoid_amd import engine
+    from meshoid_amd.batch import SnapshotBatch
+    from oracle import oracle as orc
+
+    sizes = [6000, 15000, 700]
+    ks = [32, 64, 32]
+    pos, fld = _snapshots(torch, sizes, K, seed=31 + K)
+    say("batch mfma", K, sizes)
+    B = SnapshotBatch(sizes, ks, K, nbranch=2)
+    B.force(candidate=3, nbranch=2)
+    o = B.run(pos, fld, graph=False)
+    info = B.info()
+    say("batch info", info)
+    assert info["choice"]["kernel"] == "mfma" and info["choice"]["param"] == K and info["branches"] == 2
+    rng = np.random.default_rng(K)
+    for s in range(len(sizes)):
+        h, rho, d1, st, cond = _single(pos[s], fld[s], ks[s])
+        assert same(o["h"][s], h) and same(o["rho"][s], rho) and same(o["status"][s], st), s
+        ix = engine.SpatialIndex().build(pos[s])
+        out = engine.knn_hsml(ix, ks[s], keep_dist=True)
+        ngb = out["nlist"].export()[0].cpu().numpy().astype(np.int64)
+        h64 = out["nlist"].h64().cpu().numpy()
+        p = pos[s].cpu().numpy().astype(np.float64)
+        f = fld[s].cpu().numpy().astype(np.float64)
+        idx = np.sort(rng.choice(sizes[s], min(1500, sizes[s]), replace=False))
+        G = orc.gradient_weights(p, ngb[idx], h64[idx], idx)  # (m, 3, k)
+        df = np.abs(f[ngb[idx]] - f[idx][:, None, :])  # (m, k, K)
+        S = np.einsum("mcj,mjf->mfc", np.abs(G), df)  # (m, K, 3)
+        g1 = o["grad"][s].cpu().numpy().astype(np.float64)[idx]
+        g2 = d1.cpu().numpy().astype(np.float64)[idx]
+        ok = np.isfinite(g2).all(axis=(1, 2))
+        assert (np.isfinite(g1).all(axis=(1, 2)) == ok).all(), s
+        bound = (ks[s] + 4) * 2.0 ** -24 * S + 1e-7 * np.abs(g2)
+        err = np.abs(g1 - g2)
+        say("snapshot", s, "max err / bound", (err[ok] / bound[ok]).max())
+        assert (err[ok] <= bound[ok]).all(), s
+
+
+def test_batch_branch_counts_give_identical_results():
+    """The branch streams only reorder independent work: 1, 2 or 4 branches
+    give bit-identical results."""
+    torch = _torch()
+    from meshoid_amd.batch import SnapshotBatch
+
+    sizes = [4000, 9000, 1500, 6000, 300]
+    ks = [32, 64, 32, 32, 16]
+    K = 8
+    pos, fld = _snapshots(torch, sizes, K, seed=41)
+    res = []
+    for nb in (1, 2, 4):
+        say("branches", nb)
+        B = SnapshotBatch(sizes, ks, K, nbranch=4)
+        B.force(candidate=0, nbranch=nb)
+        o = B.run(pos, fld, graph=False)
+        assert B.info()["branches"] == nb
+        res.append({k: [t.clone() for t in v] for k, v in o.items()})
+    for r in res[1:]:
+        for k in r:
+            for s in range(len(sizes)):
+                assert same(r[k][s], res[0][k][s]), (k, s)
