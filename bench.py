@@ -426,7 +426,7 @@ def main():
     if model is not None:
         # autotune the search-radius factor (untimed trial runs)
         trials = {}
-        for beta in (1.15, 1.3, 1.5, 1.8, 2.2):
+        for beta in (1.0, 1.15, 1.3, 1.5, 1.8, 2.2):
             trials[beta] = timed(lambda: step_learned(beta), 2, 1, 0)[0]
         beta = min(trials, key=trials.get)
         log("learned form: radius factor trials (ms)", {b: round(t * 1e3, 2) for b, t in trials.items()})
@@ -449,6 +449,7 @@ def main():
         learned = {"particles_per_s": n / dt_l, "ms_per_step": dt_l * 1e3,
                    "newton_iterations": newton,
                    "speedup_vs_plain": dt_plain / dt_l, "radius_factor": beta,
+                   "radius_factor_trials_ms": {str(b): round(t * 1e3, 3) for b, t in trials.items()},
                    "radius_misses": int(((outl["status"] & 128) != 0).sum()),
                    "newton_seed_rejected": int(((outl["status"] & 32) != 0).sum()),
                    "seed_abs_log_err_median": float(lerr.median()),

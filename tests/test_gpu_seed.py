@@ -151,3 +151,69 @@ def test_training_learns_and_seeded_search_is_exact(oracle):
         assert torch.equal(n0, n1) and torch.equal(d0, d1)
         assert float((out["h"] / plain["h"] - 1).abs().max()) < 1e-6
         assert torch.equal(out["rho"] > 0, plain["rho"] > 0)
+
+
+def test_seeded_vs_plain_counters_2e20():
+    """Learned seed against the plain method on 2^20 clustered particles, with
+    the kNN work counters and times of each form (printed): the plain form,
+    the MLP's radius + Newton seeds at beta = 1.3 and 2.2, and an exact radius
+    seed (1.0001 x each particle's own (k+1)-th neighbour distance from the
+    plain run: the most any radius guess can give).  Asserts that every form
+    returns the plain results, that the counters add up (one count per query
+    wavefront; a miss is flagged on exactly the lanes that were recomputed),
+    and that an exact radius cuts the insertion steps (so a radius seed acts
+    as intended and its accuracy is what limits the gain)."""
+    torch = _torch()
+    import time
+
+    from meshoid_amd import engine, synthetic
+    from meshoid_amd import seed as seedmod
+
+    n, k = 1 << 20, 32
+    pos = synthetic.clustered(n, seed=4, nclump=256, background=0.1)
+    ix = engine.SpatialIndex().build(torch.from_numpy(pos).cuda())
+    model = seedmod.load_default(k)
+    hp = model.predict(ix)
+    # each particle's (k+1)-th neighbour distance: the last of a (k+1)-search
+    base = engine.knn_hsml(ix, k + 1, keep_dist=True)
+    _, d = base["nlist"].export(want_ngb=False)
+    torch.cuda.synchronize()
+    r_exact = (d[:, -1] * 1.0001).contiguous()
+    del base
+
+    def run(seed_h=None, seed_r=None):
+        engine.prof_read()
+        engine.prof_enable(2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = engine.knn_hsml(ix, k, keep_dist=True, seed_h=seed_h, seed_r=seed_r)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        prof = engine.prof_read()
+        engine.prof_enable(0)
+        st = {nm[5:]: c for nm, (c, _) in prof.items() if nm.startswith("stat_")}
+        return out, ms, st
+
+    forms = {"plain": run(), "mlp_1.3": run(hp, (1.3 * hp).contiguous()),
+             "mlp_2.2": run(hp, (2.2 * hp).contiguous()), "exact_radius": run(None, r_exact)}
+    a, _ = forms["plain"][0]["nlist"].export(want_dist=False, f64=True)
+    a = torch.sort(a, dim=1).values
+    nw = (n + 63) // 64
+    for name, (out, ms, st) in forms.items():
+        miss = int(((out["status"] & 128) != 0).sum())
+        say(name, f"{ms:.2f} ms", "misses", miss, {c: st.get(c, 0) for c in
+            ("waves", "p1_leaves", "p1_inserts", "p1_nodes", "newton", "newton_wmax")})
+        b, _ = out["nlist"].export(want_dist=False, f64=True)
+        assert int((a != torch.sort(b, dim=1).values).any(dim=1).sum()) == 0, name
+        assert float((out["h"] / forms["plain"][0]["h"] - 1).abs().max()) <= 1e-6, name
+        if name == "plain":
+            assert st["waves"] == nw and miss == 0
+        else:
+            # the redo launch counts the wavefronts it recomputes as well
+            assert nw <= st["waves"] <= 2 * nw, name
+        # at least one Newton iteration per query, at most 40
+        assert n <= st["newton"] <= 40 * st["waves"] * 64, name
+    st_p, st_e = forms["plain"][2], forms["exact_radius"][2]
+    assert int(((forms["exact_radius"][0]["status"] & 128) != 0).sum()) == 0
+    assert st_e["waves"] == nw
+    assert st_e["p1_inserts"] < 0.8 * st_p["p1_inserts"]
