@@ -42,8 +42,8 @@ def log(*a):
 def alg_bytes(region: str, k: int, K: int) -> float:
     """Algorithmic bytes per particle of each launch region: own inputs + own
     outputs + one read of every (particle, neighbour) pair."""
-    if region == "knn_main":  # own pos + k neighbour positions; h, rho, vol, status + k indices
-        return 12 + 12 * k + 16 + 4 * k
+    if region == "knn_main":  # B_alg(k, K=0): own pos 12 + mass 4 + h, rho 8 + k neighbour positions
+        return 12 + 4 + 8 + 12 * k
     if region == "knn_pass1":  # own pos + k neighbour positions + h, rho
         return 12 + 12 * k + 8
     if region in ("apply_D1", "apply_D2"):  # own pos, h, field; k x (pos, field); grad
