@@ -168,7 +168,16 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t *__restrict__ counts, 
 // Stable scatter of one tile: keys are ranked into tile-local digit order in
 // LDS first, then written out in that order, so that the threads of a wave
 // write runs of consecutive positions of one digit (coalesced) instead of 64
-// scattered 12-byte writes.
+// scattered 12-byte writes.  Each wavefront ranks a contiguous quarter of the
+// tile on its own: per round it matches its 64 keys' digits with 8 ballots
+// and advances its own digit counters in LDS, in program order (no block
+// barrier inside the rounds; an earlier form ranked block-wide rounds with
+// three barriers each).  The quarters are combined afterwards: a key's
+// tile-local position is its digit's base in the tile + the counts of that
+// digit in the earlier quarters + its rank within its quarter, which keeps
+// the order of equal digits (the sort is stable).
+constexpr int RS_WAVES = RS_THREADS / WAVE;
+constexpr int RS_QUARTER = RS_TILE / RS_WAVES;  // keys per wavefront
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ vin, int64_t n,
                                                            int shift, const uint32_t *__restrict__ counts,
@@ -177,29 +186,58 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
                                                            uint32_t *__restrict__ vout) {
     __shared__ uint64_t s_key[RS_TILE];
     __shared__ uint32_t s_val[RS_TILE];
-    __shared__ uint32_t s_gb[256];   // global position of the tile's first key of each digit
-    __shared__ uint32_t s_lb[256];   // tile-local position of the first key of each digit
-    __shared__ uint32_t s_run[256];  // running tile-local position per digit
-    __shared__ uint32_t s_w[4][256];
+    __shared__ uint32_t s_wc[RS_WAVES][256];  // per-wave digit counts (running, then offsets)
+    __shared__ uint32_t s_gb[256];            // global position of the tile's first key of each digit
+    __shared__ uint32_t s_lb[256];            // tile-local position of the first key of each digit
     __shared__ uint32_t s_tmp[256];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     const int tn = (int)min<int64_t>(RS_TILE, n - base);
-    // the tile's keys in registers, and its digit histogram
-    uint64_t key[RS_ITEMS];
-    uint32_t val[RS_ITEMS];
-    s_run[tid] = 0;
+#pragma unroll
+    for (int ww = 0; ww < RS_WAVES; ww++) s_wc[ww][tid] = 0;
     __syncthreads();
+    // this wavefront's quarter: input indices w*QUARTER + r*64 + lane
+    uint64_t key[RS_ITEMS];
+    uint32_t val[RS_ITEMS], rk[RS_ITEMS];
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        const int i = r * RS_THREADS + tid;
-        key[r] = i < tn ? kin[base + i] : 0ull;
-        val[r] = i < tn ? vin[base + i] : 0u;
-        if (i < tn) atomicAdd(&s_run[(uint32_t)(key[r] >> shift) & 255u], 1u);
+        const int i = w * RS_QUARTER + r * WAVE + lane;
+        const bool valid = i < tn;
+        key[r] = valid ? kin[base + i] : 0ull;
+        val[r] = valid ? vin[base + i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int i = w * RS_QUARTER + r * WAVE + lane;
+        const bool valid = i < tn;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bb = 0; bb < 8; bb++) {
+            const bool bit = (d >> bb) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll(peers & lt_mask);
+        // every lane reads its digit's count before the lowest lane of each
+        // digit group advances it (one wavefront: reads, then writes, in order)
+        const uint32_t before = valid ? s_wc[w][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) s_wc[w][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rk[r] = before + rank;
     }
     __syncthreads();
+    // per digit (thread tid = digit): offsets of the quarters, tile count
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int ww = 0; ww < RS_WAVES; ww++) {
+        const uint32_t c = s_wc[ww][tid];
+        s_wc[ww][tid] = cnt;
+        cnt += c;
+    }
     // exclusive scans over the digits: tile-local bases and global bases
-    const uint32_t cnt = s_run[tid];
     s_tmp[tid] = cnt;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
@@ -209,7 +247,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
         __syncthreads();
     }
     s_lb[tid] = s_tmp[tid] - cnt;
-    s_run[tid] = s_tmp[tid] - cnt;
     __syncthreads();
     s_tmp[tid] = totals[tid];
     __syncthreads();
@@ -220,36 +257,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
         __syncthreads();
     }
     s_gb[tid] = s_tmp[tid] - totals[tid] + counts[(int64_t)tid * nblocks + blockIdx.x];
-    for (int i = tid; i < 1024; i += RS_THREADS) (&s_w[0][0])[i] = 0;
     __syncthreads();
-    // stable ranking into tile-local order: rounds in input order, lanes by
-    // ballot matching within a wave, waves in order within a round
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        const bool valid = r * RS_THREADS + tid < tn;
-        const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int bb = 0; bb < 8; bb++) {
-            const bool bit = (d >> bb) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        const uint32_t rank = __popcll(peers & lt_mask);
-        if (valid && rank == 0) s_w[w][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = s_run[d] + rank;
-            for (int ww = 0; ww < w; ww++) pos += s_w[ww][d];
+        const int i = w * RS_QUARTER + r * WAVE + lane;
+        if (i < tn) {
+            const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+            const uint32_t pos = s_lb[d] + s_wc[w][d] + rk[r];
             s_key[pos] = key[r];
             s_val[pos] = val[r];
         }
-        __syncthreads();
-        s_run[tid] += s_w[0][tid] + s_w[1][tid] + s_w[2][tid] + s_w[3][tid];
-        s_w[0][tid] = s_w[1][tid] = s_w[2][tid] = s_w[3][tid] = 0;
-        __syncthreads();
     }
+    __syncthreads();
     // write out in tile-local digit order: runs of one digit are contiguous
     for (int i = tid; i < tn; i += RS_THREADS) {
         const uint64_t k = s_key[i];
