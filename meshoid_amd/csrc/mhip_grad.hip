@@ -51,8 +51,9 @@ MHIP_DEV void pair_offset(const ApplyArgs &a, const float (&c)[3], const float (
 // wavefronts per SIMD (~230 registers), so a loop that waits on one
 // neighbour's dependent loads at a time stalls (measured 6.25 -> 5.24 ms at
 // 2^24, K = 2, with batches of 4; 7.2 ms with 8).  First-order fits keep one
-// neighbour per batch: their four lanes per query and higher occupancy hide
-// the latency already, and batching cost occupancy (2.61 -> 2.93 ms with 4).
+// neighbour per batch: their lanes and higher occupancy hide the latency
+// already, and batching cost occupancy (with the packed loads and two lanes
+// per query: 1.97 ms, against 2.31 with 2 and 2.42 with 4 per batch).
 template <int ORDER>
 constexpr int apply_nb() { return ORDER == 1 ? 1 : 4; }
 
