@@ -37,6 +37,7 @@
 //           so neighbour sets equal the float64 kNN except for float64 ties.
 #include "../../include/meshoid_hip.h"
 #include "mhip_internal.hpp"
+#include "mhip_sortnet.hpp"
 
 namespace mhip {
 
@@ -527,6 +528,32 @@ __global__ __launch_bounds__(KNN_BLOCK) void k_knn_main(KnnArgs a) {
     uint32_t x = 0, off = 0;  // node whose children (level lvl + 1) are pending; its offset in its level
     int64_t L = l0;           // next own leaf
     bool walking = false, done = false;
+    if (KCAP == 33 && e == 0) {
+        // k = 32: the own leaves fill the lists through sorting networks, one
+        // leaf at a time (its 16 distances sorted in registers, then merged
+        // into the 33-slot list; the merge's other 16 inputs are +inf),
+        // instead of one queued insertion step per candidate (64 steps of 34
+        // instructions for four own leaves)
+        for (int64_t li = l0; li <= l1; li++) {
+            load_leaf_uniform<P64>(sx, sy, sz, lx, ly, lz, li, cx, cy, cz, clx, cly, clz);
+            float S[32];
+#pragma unroll
+            for (int i = 0; i < LEAF; i++) {
+                const float d2 = dist2<PER, P64>(t, q, cx[i], cy[i], cz[i], clx[i], cly[i], clz[i]);
+                S[i] = fminf(d2, INFINITY);  // padding (NaN) -> +inf
+            }
+            oddeven_sort<LEAF>(*reinterpret_cast<float(*)[LEAF]>(&S[0]));
+#pragma unroll
+            for (int i = LEAF; i < 32; i++) S[i] = INFINITY;
+            merge_into<32>(*reinterpret_cast<float(*)[33]>(&A[0]), S);
+            if (nwl < a.wlcap && lane == 0) s_wl[w][nwl] = (uint32_t)li;
+            nwl++;
+            n_leaf++;
+        }
+        L = l1 + 1;
+        t1 = A[KCAP - 1];
+        bound = t1 * PRUNE_REL;
+    }
     // batches: scan leaves until some queue is nearly full (the sorted lists
     // are not touched there), then drain every queue into its list
     while (true) {
