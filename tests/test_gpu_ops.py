@@ -197,3 +197,46 @@ def test_linear_exactness_clustered_large():
     err = (d1[:, 0, :].double() - b).norm(dim=1) / b.norm()
     say("linear exactness max rel err", float(err.max()), "median", float(err.median()))
     assert float(torch.quantile(err[:1 << 20], 0.999)) < 1e-5
+
+
+@pytest.mark.parametrize("k,box,weighted,masked", [(32, None, True, False), (33, None, True, False),
+                                                   (2, None, True, False), (32, 10.0, True, False),
+                                                   (32, None, False, False), (32, None, True, True)])
+def test_single_field_path_equals_generic(k, box, weighted, masked):
+    """One float32 field in 3D (order 1) takes the pipelined packed-load
+    kernel; the same field as column 0 of a two-field input takes the generic
+    kernel.  Both sum the same float64 terms, split over 4 and 2 lanes per
+    query: equal status and NaN rows, and float32 results that differ only
+    where the float64 sums' last bits cross a float32 rounding boundary."""
+    torch = _torch()
+    from meshoid_amd import engine, synthetic
+
+    say("single-field path", k, box, weighted, masked)
+    n = 150000
+    pos = synthetic.clustered(n, seed=k, nclump=16, box=10.0, background=0.1, dtype=np.float64)
+    if box:
+        box = 10.0
+        pos = np.mod(pos, box)
+    pos = pos.astype(np.float32)
+    if box:
+        pos[pos >= box] = 0.0
+    rng = np.random.default_rng(k)
+    f = rng.standard_normal(n).astype(np.float32)
+    ix = engine.SpatialIndex().build(torch.from_numpy(pos).cuda(), boxsize=box)
+    qidx = None
+    if masked:
+        qidx = torch.from_numpy(np.sort(rng.choice(n, n // 3, replace=False))).cuda()
+    nl = engine.knn_hsml(ix, k, qidx=qidx)["nlist"]
+    fd = torch.from_numpy(f).cuda()
+    d1, _, st1 = engine.apply_D(nl, fd[:, None].contiguous(), weighted=weighted)
+    d2, _, st2 = engine.apply_D(nl, torch.stack([fd, 2 * fd], 1).contiguous(), weighted=weighted)
+    torch.cuda.synchronize()
+    a = d1[:, 0].cpu().numpy()
+    b = d2[:, 0].cpu().numpy()
+    assert torch.equal(st1, st2)
+    nan = np.isnan(a).any(1)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    same = (a[~nan] == b[~nan]).mean()
+    err = rel_err(a[~nan], b[~nan]) if (~nan).any() else 0.0
+    say("bit-identical fraction", same, "max row rel diff", err)
+    assert same > 0.999 and err < 1e-5
