@@ -341,8 +341,9 @@ int mhip_reconstruct(mhip_ctx *ctx, const double *points, int64_t nt, int dim, d
  * snapshots) is captured once into a HIP graph and replayed (recaptured when
  * the buffers change).  The first run is eager and times the number of
  * branches in use (1, 2 or 4, at most nbranch) and then the contraction's
- * candidates (K >= 8: a group of 16/32/64 threads per query over row-major
- * fields, wide = 1, plus, for K = 16, 32 or 64, the matrix-core kernel;
+ * candidates (K >= 8: a group of 8/16/32 threads per query over row-major
+ * fields, each thread summing 4/2/1 fields per pass, wide = 1, plus, for
+ * K = 16, 32 or 64, the matrix-core kernel;
  * fewer fields: 2/4/8/16 fields per thread); mhip_batch_info and
  * mhip_batch_tuning report them and the choices.  Non-periodic sets only.
  */

@@ -18,8 +18,9 @@
 // bbox / NaN records are validated after each run.
 //
 // The contraction kernel: with K >= 8 fields, k_apply_wide (a thread group
-// per query, row-major fields; group width 16, 32 or 64 threads) and, when K
-// is 16, 32 or 64, k_apply_mfma (the same with the field contraction on the
+// per query, row-major fields; group width 8, 16 or 32 threads, each thread
+// summing up to 4, 2 or 1 fields per pass) and, when K is 16, 32 or 64,
+// k_apply_mfma (the same with the field contraction on the
 // matrix cores); with fewer, the tile kernel (2, 4, 8 or 16 fields per
 // thread).  The first eager run times the number of branch streams in use
 // (1, 2 or 4, at most the nbranch given: the device runs 4 hardware queues
@@ -70,7 +71,7 @@ struct Cand {
     int kind, param, qpb;
 };
 static const Cand TILE_CAND[4] = {{CAND_TILE, 2, 256}, {CAND_TILE, 4, 256}, {CAND_TILE, 8, 256}, {CAND_TILE, 16, 256}};
-static const Cand WIDE_CAND[3] = {{CAND_WIDE, 16, 16}, {CAND_WIDE, 32, 8}, {CAND_WIDE, 64, 4}};
+static const Cand WIDE_CAND[3] = {{CAND_WIDE, 8, 16}, {CAND_WIDE, 16, 16}, {CAND_WIDE, 32, 8}};
 static bool has_mfma(const mhip_batch *b) { return b->wide && (b->K == 16 || b->K == 32 || b->K == 64); }
 static int ncand(const mhip_batch *b) { return b->wide ? (has_mfma(b) ? 4 : 3) : 4; }
 static Cand cand(const mhip_batch *b, int c) {
@@ -83,7 +84,10 @@ static int launch_contraction(mhip_ctx *ctx, mhip_batch *b, int c) {
     const int64_t *blk0 = b->blk0_d + (int64_t)c * (b->nsnap + 1);
     const Cand cd = cand(b, c);
     if (cd.kind == CAND_MFMA) return apply_mfma_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cd.param);
-    if (cd.kind == CAND_WIDE) return apply_wide_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cd.param);
+    if (cd.kind == CAND_WIDE) {
+        const int kmax = *std::max_element(b->k.begin(), b->k.end());
+        return apply_wide_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cd.param, b->K, kmax);
+    }
     return apply_batch_launch(ctx, b->tab_d, blk0, b->nsnap, b->nblocks[c], cd.param, b->K);
 }
 

@@ -13,6 +13,7 @@
 // field, then solves with an unpivoted float64 LDL^T (E is symmetric positive
 // semi-definite; an exactly zero or NaN pivot is reported as singular, as
 // np.linalg.inv raises on an exactly zero pivot).
+#include <algorithm>
 #include <type_traits>
 
 #include "mhip_internal.hpp"
@@ -371,6 +372,8 @@ __global__ __launch_bounds__(256) void k_apply_batch(const ApplyArgs *__restrict
 // one neighbour's KW consecutive field values (one cache line) together.
 // Same operator as apply_row, summed in another order.
 constexpr int WIDE_KMAX = 128;
+// queries per block with KW threads each: 256 threads, or 16 queries of 8
+constexpr int wide_g(int KW) { return KW >= 16 ? 256 / KW : 16; }
 
 template <int KW>
 struct WideQuery {
@@ -384,8 +387,8 @@ struct WideQuery {
 // the block's snapshot (binary search of the block prefix) and phase 1
 template <int KW>
 MHIP_DEV WideQuery<KW> wide_phase1(const ApplyArgs *__restrict__ tab, const int64_t *__restrict__ blk0, int nsnap,
-                                   double (*s_g)[WIDE_KMAX][3], uint32_t (*s_p)[WIDE_KMAX], const ApplyArgs *&ap) {
-    constexpr int G = 256 / KW;
+                                   double *s_g, uint32_t *s_p, int kmax, const ApplyArgs *&ap) {
+    constexpr int G = wide_g(KW);
     const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
     int lo = 0, hi = nsnap;
     while (hi - lo > 1) {
@@ -410,7 +413,7 @@ MHIP_DEV WideQuery<KW> wide_phase1(const ApplyArgs *__restrict__ tab, const int6
     for (int j = f; j < k; j += KW) {
         const uint32_t p = a.nl_idx[(int64_t)j * a.nq + w.sc];
         const double d[3] = {(double)a.sx[p] - qx, (double)a.sy[p] - qy, (double)a.sz[p] - qz};
-        const double wt = kernel_w<double>(sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
+        const double wt = kernel_w<double>(pair_dist(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
         double wa[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) wa[i] = wt * d[i];
@@ -420,8 +423,8 @@ MHIP_DEV WideQuery<KW> wide_phase1(const ApplyArgs *__restrict__ tab, const int6
             for (int jj = 0; jj <= i; jj++) E[tri(i, jj)] = fma(wa[i], d[jj], E[tri(i, jj)]);
         }
 #pragma unroll
-        for (int i = 0; i < 3; i++) s_g[g][j][i] = wa[i];
-        s_p[g][j] = p;
+        for (int i = 0; i < 3; i++) s_g[((int64_t)g * kmax + j) * 3 + i] = wa[i];
+        s_p[g * kmax + j] = p;
     }
 #pragma unroll
     for (int o = KW / 2; o > 0; o >>= 1) {
@@ -434,45 +437,110 @@ MHIP_DEV WideQuery<KW> wide_phase1(const ApplyArgs *__restrict__ tab, const int6
     if (!w.ok) w.st |= ST_RANKDEF;
     __syncthreads();
     for (int j = f; j < k; j += KW) {
-        double v[3] = {s_g[g][j][0], s_g[g][j][1], s_g[g][j][2]};
+        double *gj = s_g + ((int64_t)g * kmax + j) * 3;
+        double v[3] = {gj[0], gj[1], gj[2]};
         ldlt_solve<3>(E, v);
 #pragma unroll
-        for (int i = 0; i < 3; i++) s_g[g][j][i] = v[i];
+        for (int i = 0; i < 3; i++) gj[i] = v[i];
     }
     __syncthreads();
     return w;
 }
 
-template <int KW>
-__global__ __launch_bounds__(256) void k_apply_wide(const ApplyArgs *__restrict__ tab,
-                                                    const int64_t *__restrict__ blk0, int nsnap) {
-    constexpr int G = 256 / KW;
-    __shared__ double s_g[G][WIDE_KMAX][3];
-    __shared__ uint32_t s_p[G][WIDE_KMAX];
+// Phase 2 with FPL fields per thread: thread f of a query sums FPL
+// consecutive fields of each pass (one 4/8/16-byte load per neighbour when K
+// is a multiple of FPL, else FPL single loads), so the neighbour's index and
+// float64 weights are read from LDS once for FPL fields.  Every output is the
+// same sequential float64 sum over j whatever KW and FPL are.  The LDS holds
+// kmax neighbours per query (dynamic: the largest des_ngb of the batch).
+template <int FPL>
+struct FieldVec;
+template <>
+struct FieldVec<1> {
+    typedef float T;
+    static MHIP_DEV float at(const T &v, int) { return v; }
+};
+template <>
+struct FieldVec<2> {
+    typedef float2 T;
+    static MHIP_DEV float at(const T &v, int i) { return i == 0 ? v.x : v.y; }
+};
+template <>
+struct FieldVec<4> {
+    typedef float4 T;
+    static MHIP_DEV float at(const T &v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+};
+
+template <int FPL, bool VEC>
+MHIP_DEV void wide_pass(const float *__restrict__ fs, int K, const double *sg, const uint32_t *sp, int k,
+                        uint32_t qp, int c, double (&o)[FPL][3]) {
+    // c: this thread's first field of the pass (c + FPL <= K when VEC)
+    double fi[FPL];
+    int col[FPL];
+#pragma unroll
+    for (int i = 0; i < FPL; i++) {
+        col[i] = min(c + i, K - 1);
+        fi[i] = (double)fs[(int64_t)qp * K + col[i]];
+        o[i][0] = o[i][1] = o[i][2] = 0.0;
+    }
+    // unrolled so that several neighbours' field loads are in flight
+    constexpr int UNR = 8 / FPL;
+#pragma unroll UNR
+    for (int j = 0; j < k; j++) {
+        const float *row = fs + (int64_t)sp[j] * K;
+        const double g0 = sg[3 * j], g1 = sg[3 * j + 1], g2 = sg[3 * j + 2];
+        float v[FPL];
+        if (VEC) {
+            const typename FieldVec<FPL>::T x = *reinterpret_cast<const typename FieldVec<FPL>::T *>(row + c);
+#pragma unroll
+            for (int i = 0; i < FPL; i++) v[i] = FieldVec<FPL>::at(x, i);
+        } else {
+#pragma unroll
+            for (int i = 0; i < FPL; i++) v[i] = row[col[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < FPL; i++) {
+            const double df = (double)v[i] - fi[i];
+            o[i][0] = fma(g0, df, o[i][0]);
+            o[i][1] = fma(g1, df, o[i][1]);
+            o[i][2] = fma(g2, df, o[i][2]);
+        }
+    }
+}
+
+template <int KW, int FPL>
+__global__ __launch_bounds__(wide_g(KW) * KW) void k_apply_wide(const ApplyArgs *__restrict__ tab,
+                                                                const int64_t *__restrict__ blk0, int nsnap,
+                                                                int kmax) {
+    constexpr int G = wide_g(KW);
+    extern __shared__ double s_dyn[];
+    double *s_g = s_dyn;
+    uint32_t *s_p = reinterpret_cast<uint32_t *>(s_dyn + (size_t)G * kmax * 3);
     const ApplyArgs *ap;
-    const WideQuery<KW> w = wide_phase1<KW>(tab, blk0, nsnap, s_g, s_p, ap);
+    const WideQuery<KW> w = wide_phase1<KW>(tab, blk0, nsnap, s_g, s_p, kmax, ap);
     const ApplyArgs &a = *ap;
     const int g = threadIdx.x / KW, f = threadIdx.x % KW;
-    const int k = a.k;
-    // phase 2: fields f, f + KW, ...; the neighbour loop is unrolled so that
-    // several neighbours' field loads are in flight at once
+    const int k = a.k, K = a.K;
     const float *fs = (const float *)a.fs;
-    for (int c = f; c < a.K; c += KW) {
-        const double fi = (double)fs[(int64_t)w.qp * a.K + c];
-        double o0 = 0.0, o1 = 0.0, o2 = 0.0;
-#pragma unroll 8
-        for (int j = 0; j < k; j++) {
-            const double df = (double)fs[(int64_t)s_p[g][j] * a.K + c] - fi;
-            o0 = fma(s_g[g][j][0], df, o0);
-            o1 = fma(s_g[g][j][1], df, o1);
-            o2 = fma(s_g[g][j][2], df, o2);
-        }
-        if (!w.ok) o0 = o1 = o2 = (double)NAN;
-        if (w.live) {
-            float *o = a.out1 + ((int64_t)w.row * a.K + c) * 3;
-            o[0] = (float)o0;
-            o[1] = (float)o1;
-            o[2] = (float)o2;
+    const double *sg = s_g + (int64_t)g * kmax * 3;
+    const uint32_t *sp = s_p + g * kmax;
+    // vector loads need whole FPL-field groups (rows then stay aligned too)
+    const bool vec = FPL > 1 && K % FPL == 0;
+    for (int c0 = 0; c0 < K; c0 += KW * FPL) {
+        const int c = c0 + f * FPL;
+        double o[FPL][3];
+        if (vec)
+            wide_pass<FPL, true>(fs, K, sg, sp, k, w.qp, min(c, K - FPL), o);
+        else
+            wide_pass<FPL, false>(fs, K, sg, sp, k, w.qp, c, o);
+#pragma unroll
+        for (int i = 0; i < FPL; i++) {
+            if (!w.live || c + i >= K) continue;
+            if (!w.ok) o[i][0] = o[i][1] = o[i][2] = (double)NAN;
+            float *out = a.out1 + ((int64_t)w.row * K + c + i) * 3;
+            out[0] = (float)o[i][0];
+            out[1] = (float)o[i][1];
+            out[2] = (float)o[i][2];
         }
     }
     if (w.live && f == 0 && a.status) a.status[w.row] = a.or_status ? (a.status[w.row] | w.st) : w.st;
@@ -496,7 +564,7 @@ __global__ __launch_bounds__(256) void k_apply_mfma(const ApplyArgs *__restrict_
     __shared__ double s_g[G][WIDE_KMAX][3];
     __shared__ uint32_t s_p[G][WIDE_KMAX];
     const ApplyArgs *ap;
-    const WideQuery<KW> w = wide_phase1<KW>(tab, blk0, nsnap, s_g, s_p, ap);
+    const WideQuery<KW> w = wide_phase1<KW>(tab, blk0, nsnap, &s_g[0][0][0], &s_p[0][0], WIDE_KMAX, ap);
     const ApplyArgs &a = *ap;
     const int g = threadIdx.x / KW, f = threadIdx.x % KW;
     // this lane's block: its query is g (a wavefront holds 64 / KW whole
@@ -555,14 +623,23 @@ int apply_mfma_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0
 }
 
 int apply_wide_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
-                      int64_t nblocks, int kw) {
-    const dim3 g((unsigned)nblocks), b(256);
-    switch (kw) {
-        case 16: hipLaunchKernelGGL(k_apply_wide<16>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
-        case 32: hipLaunchKernelGGL(k_apply_wide<32>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
-        case 64: hipLaunchKernelGGL(k_apply_wide<64>, g, b, 0, ctx->stream, tab_d, blk0_d, nsnap); break;
-        default: return fail(ctx, -1, "unsupported field group");
-    }
+                      int64_t nblocks, int kw, int K, int kmax) {
+    if (kmax < 1 || kmax > WIDE_KMAX) return fail(ctx, -1, "des_ngb outside the wide kernel's range");
+    const dim3 g((unsigned)nblocks);
+    // fields per thread: enough for K (at most 32 per pass of the group)
+    const int fpl = std::min(32 / kw, std::max(1, (K + kw - 1) / kw));
+    const size_t lds = (size_t)wide_g(kw) * kmax * (3 * sizeof(double) + sizeof(uint32_t));
+#define MHIP_WIDE(KW_, FPL_)                                                                                \
+    hipLaunchKernelGGL((k_apply_wide<KW_, FPL_>), g, dim3(wide_g(KW_) * KW_), lds, ctx->stream, tab_d, blk0_d, \
+                       nsnap, kmax)
+    if (kw == 8 && fpl >= 3) MHIP_WIDE(8, 4);
+    else if (kw == 8 && fpl == 2) MHIP_WIDE(8, 2);
+    else if (kw == 8) MHIP_WIDE(8, 1);
+    else if (kw == 16 && fpl >= 2) MHIP_WIDE(16, 2);
+    else if (kw == 16) MHIP_WIDE(16, 1);
+    else if (kw == 32) MHIP_WIDE(32, 1);
+    else return fail(ctx, -1, "unsupported field group");
+#undef MHIP_WIDE
     return check_hip(ctx, hipGetLastError(), "wide apply launch");
 }
 

@@ -109,10 +109,11 @@ int apply_batch_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk
 void apply_args_fill(const mhip_nlist *nl, const void *fs, int K, float *out1, int32_t *status,
                      int or_status, ApplyArgs *a);
 // many fields at once (order 1, 3D, float32 positions, row-major fields
-// [npad][K], des_ngb <= 128): kw threads per query (16, 32 or 64), a block
-// holds 256 / kw queries; blk0 counts query blocks of that size
+// [npad][K], des_ngb <= kmax <= 128): kw threads per query (8, 16 or 32), a
+// block holds 256 / kw queries (16 for kw = 8); blk0 counts query blocks of
+// that size
 int apply_wide_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
-                      int64_t nblocks, int kw);
+                      int64_t nblocks, int kw, int K, int kmax);
 // the same with the field contraction on the matrix cores (K == kw fields)
 int apply_mfma_launch(mhip_ctx *ctx, const ApplyArgs *tab_d, const int64_t *blk0_d, int nsnap,
                       int64_t nblocks, int kw);

@@ -127,11 +127,35 @@ def test_batch_wide_contraction():
     o = B.run(pos, fld, graph=False)
     info = B.info()
     say("batch info", info)
-    assert info["choice"]["kernel"] == "wide" and info["choice"]["param"] in (16, 32, 64)
+    assert info["choice"]["kernel"] == "wide" and info["choice"]["param"] in (8, 16, 32)
     assert [c["kernel"] for c in info["candidates"]] == ["wide"] * 3  # K = 12: no matrix-core kernel
     for s in range(len(sizes)):
         h, rho, d1, st, cond = _single(pos[s], fld[s], ks[s])
         assert same(o["h"][s], h) and same(o["rho"][s], rho) and same(o["status"][s], st), s
+        close(o["grad"][s], d1, s, cond)
+
+
+@pytest.mark.parametrize("K", [12, 32])
+@pytest.mark.parametrize("cand", [0, 1, 2])
+def test_batch_wide_candidates(K, cand):
+    """Each thread-group width (8, 16, 32 threads per query, summing 4, 2 or
+    1 fields per pass; K = 12 leaves part of the last pass idle), forced,
+    against the single-set float64 path."""
+    torch = _torch()
+    from meshoid_amd.batch import SnapshotBatch
+
+    sizes = [6000, 15000, 700]
+    ks = [32, 64, 32]
+    pos, fld = _snapshots(torch, sizes, K, seed=51 + K)
+    say("batch wide candidate", cand, "K", K)
+    B = SnapshotBatch(sizes, ks, K, nbranch=2)
+    B.force(candidate=cand, nbranch=2)
+    o = B.run(pos, fld, graph=False)
+    info = B.info()
+    assert info["choice"]["kernel"] == "wide" and info["choice"]["param"] == (8, 16, 32)[cand]
+    for s in range(len(sizes)):
+        h, rho, d1, st, cond = _single(pos[s], fld[s], ks[s])
+        assert same(o["status"][s], st), s
         close(o["grad"][s], d1, s, cond)
 
 
