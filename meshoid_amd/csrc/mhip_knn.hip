@@ -50,7 +50,7 @@ constexpr float AMBIG_REL = 12.0f * U32;     // fp32 vs fp64 ordering ambiguity 
 constexpr float BAND_REL = 16.0f * U32;      // resolver band half-width
 constexpr float PRUNE_REL = 1.0f + 1e-5f;    // box-test safety factor
 constexpr int BAND_CAP = 32;
-constexpr int64_t RESOLVE_WAVES = 8192;      // resolver grid (one-lane wavefronts first)
+constexpr int64_t RESOLVE_WAVES = 8192;      // resolver grid (one wavefront per ambiguous slot)
 
 struct TreeView {
     const float *sx, *sy, *sz, *lx, *ly, *lz;
@@ -838,79 +838,144 @@ MHIP_DEV double exact_d2(const PT *pos, int dim, uint32_t i, uint32_t j, double 
     return s;
 }
 
-// one ambiguous slot (from the main kernel's list)
+// one ambiguous slot (from the main kernel's list), served by a whole
+// wavefront.  The walk is the depth-first one of a single lane (children in
+// index order; a node is entered when its box lies within the band's outer
+// radius), with a node's 8 children tested by 8 lanes at once and a leaf's
+// particles classified by 16 lanes at once: members and band entries are
+// taken in the order a lone lane would take them.  The band (at most BAND_CAP
+// entries, in LDS) is then ordered and completed by lane 0.
 template <bool PER, bool P64, typename PT>
-MHIP_DEV void resolve_slot(const KnnArgs &a, const PT *__restrict__ pos_in, int64_t slot) {
-    if (slot >= a.nq) return;
-    if (!(a.st_slot[slot] & ST_AMBIGUOUS)) return;
+MHIP_DEV void resolve_slot(const KnnArgs &a, const PT *__restrict__ pos_in, int64_t slot, double *s_bd,
+                           uint32_t *s_bi) {
+    const int lane = threadIdx.x;  // one wavefront per block
     const TreeView &t = a.t;
     const uint32_t qp = a.qsorted ? a.qsorted[slot] : (uint32_t)slot;
     const QPos q = load_q<P64>(t, qp);
     const float tau = a.tau[slot];
     const float band_lo = tau * (1.0f - BAND_REL), band_hi = tau * (1.0f + BAND_REL);
+    const float thr = band_hi * PRUNE_REL;
     const uint32_t qo = a.perm[qp];
     const int64_t nq = a.nq;
-    double bd[BAND_CAP];
-    uint32_t bi[BAND_CAP];
+    const uint64_t below = (1ull << lane) - 1ull;
     int nb = 0, cnt = 0;
     int32_t st = a.st_slot[slot];
     const uint32_t first_leaf = (uint32_t)t.first_leaf;
-    uint32_t x = 0;
-    bool more = true;
-    while (more) {
-        if (node_d2<PER>(t, q, x) <= band_hi * PRUNE_REL) {
-            if (x < first_leaf) {
-                x = FANOUT * x + 1;
-                continue;
+    // children of internal node x whose boxes pass, as a bit mask
+    auto expand = [&](uint32_t x) -> uint32_t {
+        const bool pass = lane < FANOUT && node_d2<PER>(t, q, FANOUT * x + 1 + lane) <= thr;
+        return (uint32_t)__ballot(pass) & 0xffu;
+    };
+    auto leaf = [&](uint32_t x) {
+        const int64_t li = (int64_t)(x - first_leaf);
+        if (li >= t.nleaf) return;
+        const int64_t p = li * LEAF + lane;
+        float d2 = INFINITY;
+        if (lane < LEAF)
+            d2 = dist2<PER, P64>(t, q, t.sx[p], t.sy[p], t.sz[p], P64 ? t.lx[p] : 0.f, P64 ? t.ly[p] : 0.f,
+                                 P64 ? t.lz[p] : 0.f);
+        const bool mem = lane < LEAF && d2 < band_lo;
+        const bool inb = lane < LEAF && !mem && d2 <= band_hi && p < a.n;
+        const uint64_t bm = __ballot(mem);
+        if (mem) {
+            const int pos = cnt + __popcll(bm & below);
+            if (pos < a.k) {
+                a.nl_idx[(int64_t)pos * nq + slot] = (uint32_t)p;
+                if (a.nl_d2) a.nl_d2[(int64_t)pos * nq + slot] = d2;
             }
-            const int64_t base = (int64_t)(x - first_leaf) * LEAF;
-            for (int i = 0; i < LEAF && x - first_leaf < t.nleaf; i++) {
-                const int64_t p = base + i;
-                const float d2 = dist2<PER, P64>(t, q, t.sx[p], t.sy[p], t.sz[p], P64 ? t.lx[p] : 0.f,
-                                                 P64 ? t.ly[p] : 0.f, P64 ? t.lz[p] : 0.f);
-                if (d2 < band_lo) {
-                    if (cnt < a.k) {
-                        a.nl_idx[(int64_t)cnt * nq + slot] = (uint32_t)p;
-                        if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = d2;
-                    }
-                    cnt++;
-                } else if (d2 <= band_hi && p < a.n) {
-                    const double e = exact_d2<PT, PER>(pos_in, a.dim, qo, a.perm[p], (double)t.box);
-                    if (nb < BAND_CAP) {
-                        bd[nb] = e;
-                        bi[nb] = (uint32_t)p;
-                        nb++;
-                    } else {
-                        st |= ST_BAND_OVERFLOW;
-                        int wmax = 0;  // replace the largest entry if this one is smaller
-                        for (int m = 1; m < BAND_CAP; m++)
-                            if (bd[m] > bd[wmax]) wmax = m;
-                        if (e < bd[wmax]) {
-                            bd[wmax] = e;
-                            bi[wmax] = (uint32_t)p;
-                        }
+        }
+        cnt += __popcll(bm);
+        const uint64_t bb = __ballot(inb);
+        if (!bb) return;
+        const double e = inb ? exact_d2<PT, PER>(pos_in, a.dim, qo, a.perm[p], (double)t.box) : 0.0;
+        const int nbb = __popcll(bb);
+        if (nb + nbb <= BAND_CAP) {
+            if (inb) {
+                const int i = nb + __popcll(bb & below);
+                s_bd[i] = e;
+                s_bi[i] = (uint32_t)p;
+            }
+            nb += nbb;
+            __syncthreads();
+            return;
+        }
+        // entry by entry: appended while there is room, then replacing the
+        // largest entry when smaller
+        __syncthreads();
+        uint64_t rem = bb;
+        while (rem) {
+            const int l = __ffsll((long long)rem) - 1;
+            rem &= rem - 1ull;
+            const double el = __shfl(e, l);
+            const uint32_t pl = (uint32_t)__shfl((int)p, l);
+            if (lane == 0) {
+                if (nb < BAND_CAP) {
+                    s_bd[nb] = el;
+                    s_bi[nb] = pl;
+                } else {
+                    int wmax = 0;
+                    for (int m = 1; m < BAND_CAP; m++)
+                        if (s_bd[m] > s_bd[wmax]) wmax = m;
+                    if (el < s_bd[wmax]) {
+                        s_bd[wmax] = el;
+                        s_bi[wmax] = pl;
                     }
                 }
             }
+            if (nb < BAND_CAP)
+                nb++;
+            else
+                st |= ST_BAND_OVERFLOW;
+            __syncthreads();
         }
-        more = next_after(x);
+    };
+    if (node_d2<PER>(t, q, 0) <= thr) {
+        if (first_leaf == 0) {
+            leaf(0);
+        } else {
+            MaskStack ms;
+            ms.put(0, expand(0));
+            int lvl = 0;
+            uint32_t x = 0;
+            while (true) {
+                const uint32_t m = ms.get(lvl);
+                if (m == 0u) {
+                    if (lvl == 0) break;
+                    lvl--;
+                    x = (x - 1) >> 3;
+                    continue;
+                }
+                const uint32_t c = (uint32_t)__builtin_ctz(m);
+                ms.clear(lvl, c);
+                const uint32_t y = FANOUT * x + 1 + c;
+                if (y >= first_leaf) {
+                    leaf(y);
+                    continue;
+                }
+                lvl++;
+                x = y;
+                ms.put(lvl, expand(y));
+            }
+        }
     }
+    __syncthreads();
+    if (lane != 0) return;
     // order the band by (exact d^2, original index) and take what is missing
     for (int i = 1; i < nb; i++) {
-        double v = bd[i];
-        uint32_t vi = bi[i];
+        const double v = s_bd[i];
+        const uint32_t vi = s_bi[i];
         int j = i - 1;
-        while (j >= 0 && (bd[j] > v || (bd[j] == v && a.perm[bi[j]] > a.perm[vi]))) {
-            bd[j + 1] = bd[j];
-            bi[j + 1] = bi[j];
+        while (j >= 0 && (s_bd[j] > v || (s_bd[j] == v && a.perm[s_bi[j]] > a.perm[vi]))) {
+            s_bd[j + 1] = s_bd[j];
+            s_bi[j + 1] = s_bi[j];
             j--;
         }
-        bd[j + 1] = v;
-        bi[j + 1] = vi;
+        s_bd[j + 1] = v;
+        s_bi[j + 1] = vi;
     }
     for (int i = 0; i < nb && cnt < a.k; i++, cnt++) {
-        a.nl_idx[(int64_t)cnt * nq + slot] = bi[i];
-        if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = (float)bd[i];
+        a.nl_idx[(int64_t)cnt * nq + slot] = s_bi[i];
+        if (a.nl_d2) a.nl_d2[(int64_t)cnt * nq + slot] = (float)s_bd[i];
     }
     if (cnt != a.k) {
         st |= ST_COUNT;
@@ -932,16 +997,19 @@ __global__ void k_amb_reset(uint32_t *__restrict__ amb) {
     if (threadIdx.x == 0) amb[0] = 0u;
 }
 
-// the list of ambiguous slots, spread one lane per wavefront first: entry i
-// goes to wavefront i mod W, lane (i / W) mod 64.  Each slot is a long
-// divergent walk, so a short list is best served by many wavefronts with one
-// busy lane each; lanes fill up only when the list is longer than the grid.
+// the list of ambiguous slots, one wavefront per slot (a grid-stride loop
+// over the list, whose length only the device knows)
 template <bool PER, bool P64, typename PT>
 __global__ __launch_bounds__(64) void k_knn_resolve(KnnArgs a, const PT *__restrict__ pos_in) {
+    __shared__ double s_bd[BAND_CAP];
+    __shared__ uint32_t s_bi[BAND_CAP];
     const uint32_t count = a.amb[0];
-    const int64_t nw = gridDim.x;
-    for (int64_t i = blockIdx.x + threadIdx.x * nw; i < count; i += 64 * nw)
-        resolve_slot<PER, P64, PT>(a, pos_in, (int64_t)a.amb[1 + i]);
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const int64_t slot = a.amb[1 + i];
+        if (slot >= a.nq || !(a.st_slot[slot] & ST_AMBIGUOUS)) continue;
+        resolve_slot<PER, P64, PT>(a, pos_in, slot, s_bd, s_bi);
+        __syncthreads();  // the next slot reuses the band arrays
+    }
 }
 
 __global__ void k_qsetup_mask(const int64_t *__restrict__ qidx, int64_t nq,

@@ -151,6 +151,32 @@ def test_knn_duplicates_and_lattice_ties(oracle):
     assert np.abs(h / h_o - 1).max() <= 1e-6
 
 
+def test_knn_band_overflow_ties(oracle):
+    """More float64 ties at the k-th distance than the resolver's band holds:
+    8 copies of every point of a 6^3 lattice give an interior particle 8
+    members at distance 0 and 48 candidates tied at distance 1 for the
+    remaining 24 places.  The overflow is flagged, and the sets and distances
+    still agree with the oracle up to those ties."""
+    from meshoid_amd._lib import ST_BAND_OVERFLOW
+
+    g = np.arange(6, dtype=np.float64)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = np.repeat(lat, 8, axis=0)
+    say("band overflow", len(pos))
+    ngb, dist, h, st, _ = gpu_knn(pos, 32)
+    d_o, n_o = oracle.knn(pos, 32)
+    interior = np.all((lat > 0) & (lat < 5), axis=1).repeat(8)
+    assert (st[interior] & ST_BAND_OVERFLOW).all()
+    assert set_mismatch(ngb, dist, n_o, d_o) == 0
+    np.testing.assert_allclose(dist, d_o, rtol=1e-6, atol=1e-12)
+    # every row holds its own copies and 24 particles at distance 1.  (h is
+    # not compared: 8 coincident particles alone weigh (32/3) 8 > 32, so the
+    # smoothing-length equation has no root and both solvers stop at the
+    # bottom of their brackets.)
+    assert (np.sum(dist[interior] == 0.0, axis=1) == 8).all()
+    assert (np.sum(dist[interior] == 1.0, axis=1) == 24).all()
+
+
 def test_knn_errors():
     torch = _torch()
     from meshoid_amd import engine
