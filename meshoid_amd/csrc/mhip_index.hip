@@ -280,13 +280,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
 }
 
 int radix_sort_pairs(mhip_ctx *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1,
-                     uint32_t *counts, int64_t n, int bits, uint64_t **kout, uint32_t **vout) {
+                     uint32_t *counts, int64_t n, int bits, uint64_t **kout, uint32_t **vout, int lo_bit) {
     uint64_t *ka = k0, *kb = k1;
     uint32_t *va = v0, *vb = v1;
     if (n > 1) {
         const int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
         uint32_t *totals = counts + (size_t)256 * nblocks;
-        for (int shift = 0; shift < bits; shift += 8) {
+        for (int shift = lo_bit; shift < bits; shift += 8) {
             hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, ka, n,
                                shift, counts, nblocks);
             hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, ctx->stream, counts, nblocks,
@@ -485,7 +485,9 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
                            k0, v0);
     uint64_t *ks;
     uint32_t *vs;
-    if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, KEY_BITS_PER_AXIS * dim, &ks, &vs))) return rc;
+    if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, n, KEY_BITS_PER_AXIS * dim, &ks, &vs,
+                               key_sort_lo_bit(dim))))
+        return rc;
     pr_key.end();
     ProfRegion pr_g(ctx, "index_reorder_boxes");
     const dim3 gp((unsigned)((npad + 255) / 256));

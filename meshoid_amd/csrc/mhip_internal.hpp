@@ -11,11 +11,12 @@ int fail(mhip_ctx *ctx, int code, const std::string &msg);
 int check_hip(mhip_ctx *ctx, hipError_t e, const char *what);
 void *scratch(mhip_ctx *ctx, size_t bytes, int *rc);
 
-// LSD radix sort of (u64 key, u32 value) pairs on bits [0, bits) of the key.
-// Buffers: k0/v0 hold the input; the sorted result is returned in *kout/*vout
-// (one of the two ping-pong buffers).  counts needs 256*ceil(n/4096)+256 u32.
+// Stable LSD radix sort of (u64 key, u32 value) pairs on bits [lo_bit, bits)
+// of the key (lo_bit a multiple of 8).  Buffers: k0/v0 hold the input; the
+// sorted result is returned in *kout/*vout (one of the two ping-pong
+// buffers).  counts needs 256*ceil(n/4096)+256 u32.
 int radix_sort_pairs(mhip_ctx *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1,
-                     uint32_t *counts, int64_t n, int bits, uint64_t **kout, uint32_t **vout);
+                     uint32_t *counts, int64_t n, int bits, uint64_t **kout, uint32_t **vout, int lo_bit = 0);
 size_t radix_counts_words(int64_t n);
 
 // exclusive prefix sum of in[0, n) into out (uint32: the total must stay below

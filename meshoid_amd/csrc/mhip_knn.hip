@@ -1534,17 +1534,19 @@ __global__ __launch_bounds__(256) void k_nearest(NearestArgs a) {
     qb.hz = q.z + t.slack;
     const uint32_t first_leaf = (uint32_t)t.first_leaf;
     // (1) this lane's leaf: the last leaf whose first key is at most the
-    // point's key; it and its neighbours give the lane a first bound
+    // point's key (on the bits the index is sorted on); it and its neighbours
+    // give the lane a first bound
+    const int kshift = key_sort_lo_bit(a.dim);
     uint64_t kq = 0;
     {
         uint32_t cc[3] = {0u, 0u, 0u};
         for (int c = 0; c < a.dim; c++) cc[c] = key_cell((float)tp[c], a.org[c], a.scale);
-        kq = hilbert_key(cc[0], cc[1], cc[2], a.dim);
+        kq = hilbert_key(cc[0], cc[1], cc[2], a.dim) >> kshift;
     }
     uint32_t lo = 0, hi = (uint32_t)t.nleaf;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (particle_key(t, a.org, a.scale, a.dim, (int64_t)mid * LEAF) <= kq)
+        if ((particle_key(t, a.org, a.scale, a.dim, (int64_t)mid * LEAF) >> kshift) <= kq)
             lo = mid;
         else
             hi = mid;
@@ -1703,7 +1705,9 @@ int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int6
                            ix->frame[2], ix->frame[3], k0, v0);
         uint64_t *ks;
         uint32_t *vs;
-        if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nt, KEY_BITS_PER_AXIS * dim, &ks, &vs))) return rc;
+        if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nt, KEY_BITS_PER_AXIS * dim, &ks, &vs,
+                                   key_sort_lo_bit(dim))))
+            return rc;
         a.order = vs;
     }
     ProfRegion pr(ctx, "nearest");

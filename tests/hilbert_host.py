@@ -5,6 +5,8 @@ particle's cell on the 2^21-per-axis grid of the bounding box's cubic frame
 import numpy as np
 
 BITS = 21
+# key bits the index leaves unsorted, per dimension (3D: the lowest 8-bit digit)
+SORT_LO_BIT = {1: 0, 2: 0, 3: 8}
 
 
 def hilbert_keys(cells, dim, bits=BITS):

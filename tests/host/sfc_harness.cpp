@@ -12,3 +12,5 @@ extern "C" void keys(const uint32_t *cells, long n, int dim, uint64_t *out) {
 extern "C" void cells_of(const float *v, long n, float o, float s, uint32_t *out) {
     for (long i = 0; i < n; i++) out[i] = mhip::key_cell(v[i], o, s);
 }
+
+extern "C" int sort_lo_bit(int dim) { return mhip::key_sort_lo_bit(dim); }

@@ -96,3 +96,13 @@ def test_device_cell_quantisation_matches_host(sfc):
     with np.errstate(invalid="ignore", over="ignore"):
         ref = frame_cells(np.nan_to_num(v, nan=-1.0)[:, None], [0, 0, 0, 10, 10, 10], 1)[:, 0]
     assert np.array_equal(out.astype(np.uint64), ref)
+
+
+def test_sorted_key_bits_match_host_table(sfc):
+    """The bits the index leaves unsorted (header) are the ones the GPU tests
+    drop before checking the order."""
+    from hilbert_host import SORT_LO_BIT
+
+    for dim in (1, 2, 3):
+        assert sfc.sort_lo_bit(dim) == SORT_LO_BIT[dim]
+        assert SORT_LO_BIT[dim] % 8 == 0 and SORT_LO_BIT[dim] < 21 * dim
