@@ -1456,6 +1456,7 @@ struct NearestArgs {
     int64_t nt;
     int64_t *out;      // original particle index per point, -1 for a non-finite point
     unsigned long long *stats;  // work counters, may be null
+    const uint64_t *leaf_key;   // first key of each leaf, on the bits the index is sorted on
 };
 
 template <typename PT, bool PER>
@@ -1493,6 +1494,16 @@ MHIP_DEV uint64_t particle_key(const TreeView &t, const float (&org)[3], float s
     uint32_t cc[3] = {0u, 0u, 0u};
     for (int c = 0; c < dim; c++) cc[c] = key_cell(v[c], org[c], scale);
     return hilbert_key(cc[0], cc[1], cc[2], dim);
+}
+
+// first sort key of every leaf (on the bits the index is sorted on), for the
+// points' first-leaf search
+__global__ __launch_bounds__(256) void k_leaf_keys(TreeView t, float ox, float oy, float oz, float scale, int dim,
+                                                  uint64_t *__restrict__ out) {
+    const int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (l >= t.nleaf) return;
+    const float org[3] = {ox, oy, oz};
+    out[l] = particle_key(t, org, scale, dim, l * LEAF) >> key_sort_lo_bit(dim);
 }
 
 template <bool PER, bool P64, typename PT>
@@ -1546,7 +1557,7 @@ __global__ __launch_bounds__(256) void k_nearest(NearestArgs a) {
     uint32_t lo = 0, hi = (uint32_t)t.nleaf;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if ((particle_key(t, a.org, a.scale, a.dim, (int64_t)mid * LEAF) >> kshift) <= kq)
+        if (a.leaf_key[mid] <= kq)
             lo = mid;
         else
             hi = mid;
@@ -1582,37 +1593,44 @@ __global__ __launch_bounds__(256) void k_nearest(NearestArgs a) {
     scan(x);
     // (2) wave-uniform walk, inside out: the subtree of each ancestor of the
     // middle lane's leaf in turn (parent, grandparent, ... root), minus the
-    // subtree already done, depth first and pruned by every lane's bound.
+    // subtree already done, depth first (children in index order) and pruned
+    // by every lane's bound.  A node's 8 children are tested together (their
+    // boxes loaded at once) into a mask; a leaf is tested again with the
+    // bound of the moment before it is scanned.  A leaf inside a child that
+    // the bound of the moment would have pruned lies in that child's box, so
+    // the leaf test prunes it too: the leaves scanned, and their order, are
+    // those of a walk that tests each node just before entering it.
     const float F = (1.0f + BAND_REL) * PRUNE_REL;
     uint32_t done = x;  // scanned
     while (done != 0) {
         const uint32_t top = (done - 1) >> 3;
-        uint32_t y = FANOUT * top + 1;
+        MaskStack ms;
+        ms.put(0, expand_mask<PER>(t, qb, top, best * F) & ~(1u << (done - (FANOUT * top + 1))));
+        n_nodes += FANOUT;
+        int lvl = 0;
+        uint32_t cur = top;
         while (true) {
-            if (y != done) {
-                n_nodes++;
-                if (__any(box_gap2_u<PER>(t, qb, y) <= best * F)) {
-                    if (y < first_leaf) {
-                        y = FANOUT * y + 1;
-                        continue;
-                    }
-                    if (y - first_leaf < t.nleaf) {
-                        scan(y);
-                        n_leaves++;
-                    }
-                }
+            const uint32_t m = ms.get(lvl);
+            if (m == 0u) {
+                if (lvl == 0) break;
+                lvl--;
+                cur = (cur - 1) >> 3;
+                continue;
             }
-            // next node after the subtree of y, within the subtree of top
-            bool fin = false;
-            while ((y & (FANOUT - 1)) == 0) {  // y is the last child of its parent
-                y = (y - 1) >> 3;
-                if (y == top) {
-                    fin = true;
-                    break;
+            const uint32_t c = (uint32_t)__builtin_ctz(m);
+            ms.clear(lvl, c);
+            const uint32_t y = FANOUT * cur + 1 + c;
+            if (y >= first_leaf) {
+                if (y - first_leaf < t.nleaf && __any(box_gap2_u<PER>(t, qb, y) <= best * F)) {
+                    scan(y);
+                    n_leaves++;
                 }
+                continue;
             }
-            if (fin) break;
-            y++;
+            lvl++;
+            cur = y;
+            ms.put(lvl, expand_mask<PER>(t, qb, y, best * F));
+            n_nodes += FANOUT;
         }
         done = top;
     }
@@ -1692,13 +1710,18 @@ int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int6
     const bool per = ix->box > 0, p64 = ix->pos_f64 != 0;
     const dim3 g((unsigned)((nt + 255) / 256));
     hipStream_t s = ctx->stream;
-    if (nt >= 4096 && nt <= 0xffffffffll) {
+    // scratch: the points' sort (large sets), then the leaves' first keys
+    const bool sort_pts = nt >= 4096 && nt <= 0xffffffffll;
+    const size_t cw = sort_pts ? radix_counts_words(nt) : 0;
+    const size_t pts_bytes = sort_pts ? (((size_t)nt * 24 + cw * 4 + 255) & ~(size_t)255) : 0;
+    int rc = 0;
+    char *sc = (char *)scratch(ctx, pts_bytes + (size_t)ix->nleaf * sizeof(uint64_t), &rc);
+    if (!sc) return rc;
+    uint64_t *lkey = (uint64_t *)(sc + pts_bytes);
+    a.leaf_key = lkey;
+    if (sort_pts) {
         // points in the index's Hilbert order
         ProfRegion pk(ctx, "nearest_sort");
-        int rc = 0;
-        const size_t cw = radix_counts_words(nt);
-        char *sc = (char *)scratch(ctx, (size_t)nt * 24 + cw * 4, &rc);
-        if (!sc) return rc;
         uint64_t *k0 = (uint64_t *)sc, *k1 = k0 + nt;
         uint32_t *v0 = (uint32_t *)(k1 + nt), *v1 = v0 + nt, *counts = v1 + nt;
         hipLaunchKernelGGL(k_point_keys, g, dim3(256), 0, s, targets, nt, dim, a.box, ix->frame[0], ix->frame[1],
@@ -1711,6 +1734,8 @@ int nearest_run(mhip_ctx *ctx, const mhip_index *ix, const double *targets, int6
         a.order = vs;
     }
     ProfRegion pr(ctx, "nearest");
+    hipLaunchKernelGGL(k_leaf_keys, dim3((unsigned)((ix->nleaf + 255) / 256)), dim3(256), 0, s, a.t, a.org[0],
+                       a.org[1], a.org[2], a.scale, a.dim, lkey);
     if (per) {
         if (p64)
             hipLaunchKernelGGL((k_nearest<true, true, double>), g, dim3(256), 0, s, a);
