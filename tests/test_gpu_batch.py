@@ -135,17 +135,18 @@ def test_batch_wide_contraction():
         close(o["grad"][s], d1, s, cond)
 
 
-@pytest.mark.parametrize("K", [12, 32])
+@pytest.mark.parametrize("K,kmax", [(12, 64), (32, 64), (16, 128)])
 @pytest.mark.parametrize("cand", [0, 1, 2])
-def test_batch_wide_candidates(K, cand):
+def test_batch_wide_candidates(K, kmax, cand):
     """Each thread-group width (8, 16, 32 threads per query, summing 4, 2 or
     1 fields per pass; K = 12 leaves part of the last pass idle), forced,
-    against the single-set float64 path."""
+    against the single-set float64 path; des_ngb up to 128 sizes the LDS
+    for the largest lists (57 KB per block of 16 queries)."""
     torch = _torch()
     from meshoid_amd.batch import SnapshotBatch
 
     sizes = [6000, 15000, 700]
-    ks = [32, 64, 32]
+    ks = [32, kmax, 32]
     pos, fld = _snapshots(torch, sizes, K, seed=51 + K)
     say("batch wide candidate", cand, "K", K)
     B = SnapshotBatch(sizes, ks, K, nbranch=2)
