@@ -78,6 +78,19 @@ __global__ __launch_bounds__(256) void k_pack_xyzf(const float *__restrict__ sx,
     if (p < npad) out[p] = make_float4(sx[p], sy[p], sz[p], f0[p]);
 }
 
+// the same pack straight from a one-component field in original row order
+// (the field value through the sorted -> original map; NaN past nf_rows and
+// in the padding, as k_permute_fields gives)
+__global__ __launch_bounds__(256) void k_pack_xyzf_rows(const float *__restrict__ sx, const float *__restrict__ sy,
+                                                       const float *__restrict__ sz, const float *__restrict__ f,
+                                                       int64_t nf_rows, const uint32_t *__restrict__ perm,
+                                                       int64_t n, int64_t npad, float4 *__restrict__ out) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= npad) return;
+    const uint32_t o = p < n ? perm[p] : 0xffffffffu;
+    out[p] = make_float4(sx[p], sy[p], sz[p], o < nf_rows ? f[o] : NAN);
+}
+
 // one query slot s of the list, fields [fb, fe): KC fields per pass over the
 // neighbours.  With GW > 1 a group of GW consecutive lanes serves the query:
 // lane gl takes neighbours gl, gl + GW, ..., the moment matrix and right-hand
@@ -724,20 +737,25 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     // float32 positions and fields: a packed (x, y, z, field 0) copy beside
     // the sorted-space fields, so a neighbour is one 16-byte load
     const bool pack = !f_f64 && !ix->pos_f64 && ix->dim == 3;
-    const size_t fs_bytes = ((size_t)K * ix->npad * fb + 255) & ~(size_t)255;
+    // one float32 field, first order: the pipelined kernel reads only the pack
+    const bool packed1 = pack && K == 1 && order == 1 && nl->k >= 1;
+    const size_t fs_bytes = packed1 ? 0 : ((size_t)K * ix->npad * fb + 255) & ~(size_t)255;
     char *sc = (char *)scratch(ctx, fs_bytes + (pack ? (size_t)ix->npad * 16 : 0), &rc);
     if (!sc) return rc;
     void *fs = sc;
     float4 *xyzf = pack ? (float4 *)(sc + fs_bytes) : nullptr;
     const dim3 gp((unsigned)((ix->npad + 255) / 256));
     ProfRegion prp(ctx, "apply_permute_fields");
-    if (f_f64)
+    if (packed1)
+        hipLaunchKernelGGL(k_pack_xyzf_rows, gp, dim3(256), 0, ctx->stream, ix->sx, ix->sy, ix->sz,
+                           (const float *)f, nf_rows, ix->perm, ix->n, ix->npad, xyzf);
+    else if (f_f64)
         hipLaunchKernelGGL(k_permute_fields<double>, gp, dim3(256), 0, ctx->stream, (const double *)f,
                            nf_rows, K, ix->perm, ix->n, ix->npad, (double *)fs);
     else
         hipLaunchKernelGGL(k_permute_fields<float>, gp, dim3(256), 0, ctx->stream, (const float *)f,
                            nf_rows, K, ix->perm, ix->n, ix->npad, (float *)fs);
-    if (pack)
+    if (pack && !packed1)
         hipLaunchKernelGGL(k_pack_xyzf, gp, dim3(256), 0, ctx->stream, ix->sx, ix->sy, ix->sz, (const float *)fs,
                            ix->npad, xyzf);
     ApplyArgs a;
@@ -769,9 +787,8 @@ int apply_ops(mhip_ctx *ctx, const mhip_nlist *nl, const void *f, int f_f64, int
     a.or_status = 0;
     prp.end();
     ProfRegion pra(ctx, order == 1 ? "apply_D1" : "apply_D2");
-    if (pack && K == 1 && order == 1 && nl->k >= 1) {
-        // one float32 field, first order: the pipelined packed-load kernel,
-        // four lanes per query
+    if (packed1) {
+        // the pipelined packed-load kernel, four lanes per query
         constexpr int gwp = 4;
         const dim3 g((unsigned)((nl->nq * gwp + 255) / 256));
         hipLaunchKernelGGL(k_apply_packed1<gwp>, g, dim3(256), 0, ctx->stream, a);
