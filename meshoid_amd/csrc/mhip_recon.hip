@@ -66,15 +66,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t *__r
     if (threadIdx.x == SCAN_THREADS - 1) tile_sum[blockIdx.x] = sh[threadIdx.x];
 }
 
-// exclusive scan of the tile totals in place, one block walking the tiles in
-// chunks; tile_sum[ntile] receives the grand total
+// exclusive scan of the tile totals in place, one block: each thread scans
+// SUMS_PER_THREAD consecutive totals, the block scans the thread totals
+// (one pass up to 8192 tiles, i.e. 16.7 M flags; larger inputs loop with a
+// carry); tile_sum[ntile] receives the grand total
+constexpr int SUMS_PER_THREAD = 32;
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile_sums(uint32_t *__restrict__ tile_sum, int64_t ntile) {
     __shared__ uint32_t sh[SCAN_THREADS];
     uint32_t carry = 0;
-    for (int64_t c0 = 0; c0 < ntile; c0 += SCAN_THREADS) {
-        const int64_t i = c0 + threadIdx.x;
-        const uint32_t v = i < ntile ? tile_sum[i] : 0u;
-        sh[threadIdx.x] = v;
+    for (int64_t c0 = 0; c0 < ntile; c0 += (int64_t)SCAN_THREADS * SUMS_PER_THREAD) {
+        const int64_t b = c0 + (int64_t)threadIdx.x * SUMS_PER_THREAD;
+        uint32_t v[SUMS_PER_THREAD], run = 0;
+#pragma unroll
+        for (int j = 0; j < SUMS_PER_THREAD; j++) {
+            const uint32_t t = b + j < ntile ? tile_sum[b + j] : 0u;
+            v[j] = run;
+            run += t;
+        }
+        sh[threadIdx.x] = run;
         __syncthreads();
         for (int o = 1; o < SCAN_THREADS; o <<= 1) {
             const uint32_t add = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
@@ -82,7 +91,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile_sums(uint32_t *__res
             sh[threadIdx.x] += add;
             __syncthreads();
         }
-        if (i < ntile) tile_sum[i] = carry + sh[threadIdx.x] - v;
+        const uint32_t off = carry + sh[threadIdx.x] - run;
+#pragma unroll
+        for (int j = 0; j < SUMS_PER_THREAD; j++)
+            if (b + j < ntile) tile_sum[b + j] = v[j] + off;
         carry += sh[SCAN_THREADS - 1];
         __syncthreads();
     }

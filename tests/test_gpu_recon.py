@@ -89,7 +89,9 @@ def test_unique_inverse_matches_numpy():
     from meshoid_amd import engine
 
     rng = np.random.default_rng(3)
-    for n, nt in [(1, 5), (100, 1), (5000, 20000), (300000, 70000), (2100, 2100)]:
+    # 2^24 particles fill the tile-sum scan's one pass exactly; 2e7 take two
+    for n, nt in [(1, 5), (100, 1), (5000, 20000), (300000, 70000), (2100, 2100), (1 << 24, 5000),
+                  (20000000, 300000)]:
         idx = rng.integers(0, n, nt)
         u, inv = engine.unique_inverse(torch.from_numpy(idx).cuda(), n)
         uw, iw = np.unique(idx, return_inverse=True)
