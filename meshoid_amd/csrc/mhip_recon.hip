@@ -24,18 +24,6 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t *__restrict__ a, int6
         a[i] = 0u;
 }
 
-// flags[nbr[i]] = 1; points whose index is outside [0, n) are counted in bad
-__global__ __launch_bounds__(256) void k_mark(const int64_t *__restrict__ nbr, int64_t nt, int64_t n,
-                                              uint32_t *__restrict__ flags, uint32_t *__restrict__ bad) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= nt) return;
-    const int64_t j = nbr[i];
-    if (j >= 0 && j < n)
-        flags[j] = 1u;
-    else
-        atomicAdd(bad, 1u);
-}
-
 // block-local exclusive ranks of the flags, and the block totals
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t *__restrict__ flags, int64_t n,
                                                              uint32_t *__restrict__ rank,
@@ -101,25 +89,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile_sums(uint32_t *__res
     if (threadIdx.x == 0) tile_sum[ntile] = carry;
 }
 
-__global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ flags,
-                                                 const uint32_t *__restrict__ rank,
-                                                 const uint32_t *__restrict__ tile_sum, int64_t n,
-                                                 int64_t *__restrict__ uniq) {
-    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (p >= n || !flags[p]) return;
-    uniq[tile_sum[p / SCAN_TILE] + rank[p]] = p;
-}
-
-__global__ __launch_bounds__(256) void k_inverse(const int64_t *__restrict__ nbr, int64_t nt, int64_t n,
-                                                 const uint32_t *__restrict__ rank,
-                                                 const uint32_t *__restrict__ tile_sum,
-                                                 int64_t *__restrict__ inv) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= nt) return;
-    const int64_t j = nbr[i];
-    inv[i] = (j >= 0 && j < n) ? (int64_t)(tile_sum[j / SCAN_TILE] + rank[j]) : -1;
-}
-
 __global__ __launch_bounds__(256) void k_scan_add(uint32_t *__restrict__ out, const uint32_t *__restrict__ tile_sum,
                                                   int64_t n) {
     const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
@@ -141,31 +110,82 @@ uint32_t *exclusive_scan_u32(mhip_ctx *ctx, const uint32_t *in, int64_t n, uint3
     return work + ntile;
 }
 
+// unique / inverse through a bit per particle: particle j is marked in bit
+// j % 32 of word j / 32; its rank among the marked ones is the exclusive
+// prefix of the words' popcounts plus the popcount of the lower bits of its
+// own word (ascending particle order, as np.unique)
+__global__ __launch_bounds__(256) void k_mark_bits(const int64_t *__restrict__ nbr, int64_t nt, int64_t n,
+                                                   uint32_t *__restrict__ bits, uint32_t *__restrict__ bad) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const int64_t j = nbr[i];
+    if (j >= 0 && j < n)
+        atomicOr(bits + (j >> 5), 1u << (j & 31));
+    else
+        atomicAdd(bad, 1u);
+}
+
+__global__ __launch_bounds__(256) void k_popc_words(const uint32_t *__restrict__ bits, int64_t nw,
+                                                    uint32_t *__restrict__ cnt) {
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (w < nw) cnt[w] = (uint32_t)__popc(bits[w]);
+}
+
+// the marked particles of word w, in ascending order, from position wpre[w]
+__global__ __launch_bounds__(256) void k_compact_bits(const uint32_t *__restrict__ bits,
+                                                      const uint32_t *__restrict__ wpre, int64_t nw,
+                                                      int64_t *__restrict__ uniq) {
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    uint32_t m = bits[w];
+    int64_t pos = wpre[w];
+    while (m) {
+        uniq[pos++] = w * 32 + __builtin_ctz(m);
+        m &= m - 1u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_inverse_bits(const int64_t *__restrict__ nbr, int64_t nt, int64_t n,
+                                                      const uint32_t *__restrict__ bits,
+                                                      const uint32_t *__restrict__ wpre, int64_t *__restrict__ inv) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const int64_t j = nbr[i];
+    if (j < 0 || j >= n) {
+        inv[i] = -1;
+        return;
+    }
+    const uint32_t below = bits[j >> 5] & ((1u << (j & 31)) - 1u);
+    inv[i] = (int64_t)wpre[j >> 5] + __popc(below);
+}
+
 int unique_inverse(mhip_ctx *ctx, const int64_t *nbr, int64_t nt, int64_t n, int64_t *uniq, int64_t *inv,
                    int64_t *nuniq) {
     if (nt < 0 || n < 1) return fail(ctx, -1, "bad unique shape");
     if (n > 0xffffffffll) return fail(ctx, -4, "more than 2^32 particles");
     *nuniq = 0;
     if (nt == 0) return 0;
-    const int64_t ntile = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const int64_t nw = (n + 31) / 32;
     int rc = 0;
-    uint32_t *flags = (uint32_t *)scratch(ctx, (size_t)(2 * n + ntile + 2) * 4, &rc);
-    if (!flags) return rc;
-    uint32_t *rank = flags + n, *tile_sum = rank + n, *bad = tile_sum + ntile + 1;
+    // bits, popcounts, their exclusive prefix, scan work, the bad-point count
+    const size_t sw = scan_work_words(nw);
+    uint32_t *bits = (uint32_t *)scratch(ctx, (size_t)(3 * nw + sw + 1) * 4, &rc);
+    if (!bits) return rc;
+    uint32_t *cnt = bits + nw, *wpre = cnt + nw, *work = wpre + nw, *bad = work + sw;
     hipStream_t s = ctx->stream;
     ProfRegion pr(ctx, "unique");
-    const unsigned gz = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
-    hipLaunchKernelGGL(k_zero_u32, dim3(gz), dim3(256), 0, s, flags, n);
+    const unsigned gz = (unsigned)std::min<int64_t>(4096, (nw + 255) / 256);
+    hipLaunchKernelGGL(k_zero_u32, dim3(gz), dim3(256), 0, s, bits, nw);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, bad, (int64_t)1);
-    hipLaunchKernelGGL(k_mark, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, nbr, nt, n, flags, bad);
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntile), dim3(SCAN_THREADS), 0, s, flags, n, rank, tile_sum);
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(SCAN_THREADS), 0, s, tile_sum, ntile);
-    const dim3 gn((unsigned)((n + 255) / 256)), gt((unsigned)((nt + 255) / 256));
-    hipLaunchKernelGGL(k_compact, gn, dim3(256), 0, s, flags, rank, tile_sum, n, uniq);
-    hipLaunchKernelGGL(k_inverse, gt, dim3(256), 0, s, nbr, nt, n, rank, tile_sum, inv);
+    const dim3 gw((unsigned)((nw + 255) / 256)), gt((unsigned)((nt + 255) / 256));
+    hipLaunchKernelGGL(k_mark_bits, gt, dim3(256), 0, s, nbr, nt, n, bits, bad);
+    hipLaunchKernelGGL(k_popc_words, gw, dim3(256), 0, s, bits, nw, cnt);
+    const uint32_t *total = exclusive_scan_u32(ctx, cnt, nw, wpre, work);
+    hipLaunchKernelGGL(k_compact_bits, gw, dim3(256), 0, s, bits, wpre, nw, uniq);
+    hipLaunchKernelGGL(k_inverse_bits, gt, dim3(256), 0, s, nbr, nt, n, bits, wpre, inv);
     if ((rc = check_hip(ctx, hipGetLastError(), "unique launch"))) return rc;
     uint32_t host[2] = {0u, 0u};
-    if ((rc = check_hip(ctx, hipMemcpyAsync(&host[0], tile_sum + ntile, 4, hipMemcpyDeviceToHost, s), "unique count")))
+    if ((rc = check_hip(ctx, hipMemcpyAsync(&host[0], total, 4, hipMemcpyDeviceToHost, s), "unique count")))
         return rc;
     if ((rc = check_hip(ctx, hipMemcpyAsync(&host[1], bad, 4, hipMemcpyDeviceToHost, s), "unique count")))
         return rc;
