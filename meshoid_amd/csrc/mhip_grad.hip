@@ -267,15 +267,23 @@ MHIP_DEV void apply_row_packed1(const ApplyArgs &a, int64_t s, int gl) {
     const uint32_t *__restrict__ ip = a.nl_idx + s;  // entry j at ip[j * nq]
     double E[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double B[3] = {0.0, 0.0, 0.0};
-    // clamped list positions keep every prefetch inside the row
-    int j = gl;
-    const uint32_t p0 = ip[(int64_t)min(j, k - 1) * nq];
-    float4 vn = a.xyzf[p0];
-    uint32_t pn = ip[(int64_t)min(j + GW, k - 1) * nq];
-    for (; j < k; j += GW) {
+    // list pointers advance by GW entries and stop at the row's last entry,
+    // which keeps every prefetch inside the row
+    const int64_t step = (int64_t)GW * nq;
+    const uint32_t *const plast = ip + (int64_t)(k - 1) * nq;
+    auto advance = [&](const uint32_t *p) { return p + step > plast ? plast : p + step; };
+    const uint32_t *pnext = ip + (int64_t)min(gl, k - 1) * nq;
+    float4 vn = a.xyzf[*pnext];
+    pnext = advance(pnext);
+    uint32_t pn = *pnext;
+    // two neighbours per trip: the prefetch registers alternate instead of
+    // being copied (6 waves per SIMD instead of 8, measured 1.76 -> 1.72 ms)
+#pragma unroll 2
+    for (int j = gl; j < k; j += GW) {
         const float4 v = vn;
         vn = a.xyzf[pn];
-        pn = ip[(int64_t)min(j + 2 * GW, k - 1) * nq];
+        pnext = advance(pnext);
+        pn = *pnext;
         double d[3] = {(double)v.x - qx, (double)v.y - qy, (double)v.z - qz};
         if (a.per) {
             const double box = (double)a.box;
@@ -283,7 +291,12 @@ MHIP_DEV void apply_row_packed1(const ApplyArgs &a, int64_t s, int gl) {
             for (int c = 0; c < 3; c++) d[c] = wrap<double>(d[c], box);
         }
         double w = 1.0;
-        if (a.weighted) w = kernel_w<double>(pair_dist(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih);
+        if (a.weighted) {
+            // cubic spline with 1 - q clamped at 0 (zero past q = 1)
+            const double q = pair_dist(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * ih;
+            const double om = fmax(1.0 - q, 0.0);
+            w = q <= 0.5 ? 1.0 + 6.0 * q * q * (q - 1.0) : 2.0 * om * om * om;
+        }
         double wa[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) wa[i] = w * d[i];
