@@ -228,7 +228,7 @@ int mhip_nlist_create(mhip_nlist **out) {
 
 int mhip_nlist_destroy(mhip_nlist *nl) {
     if (!nl) return MHIP_OK;
-    void *arrs[] = {nl->qsorted, nl->qrow, nl->rinv, nl->idx, nl->d2, nl->tau, nl->nlt, nl->h, nl->h64, nl->status, nl->amb};
+    void *arrs[] = {nl->qsorted, nl->qrow, nl->rowrec, nl->idx, nl->d2, nl->tau, nl->nlt, nl->h, nl->h64, nl->status, nl->amb};
     for (void *p : arrs)
         if (p) hipFree(p);
     delete nl;

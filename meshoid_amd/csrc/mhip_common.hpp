@@ -179,6 +179,14 @@ struct ProfRegion {
 };
 }  // namespace mhip
 
+namespace mhip {
+// one output row of the kNN pass (16 bytes: one write per row)
+struct __align__(16) RowRec {
+    float h, rho, vol;
+    int32_t status;
+};
+}  // namespace mhip
+
 struct mhip_index {
     int64_t n = 0, npad = 0, cap = 0;
     int dim = 3;
@@ -188,7 +196,8 @@ struct mhip_index {
     float *sx = nullptr, *sy = nullptr, *sz = nullptr;  // hi parts, npad entries
     float *lx = nullptr, *ly = nullptr, *lz = nullptr;  // lo parts (f64 inputs only)
     uint32_t *perm = nullptr;                           // sorted -> original
-    uint32_t *iperm = nullptr;                          // original -> sorted
+    uint32_t *iperm = nullptr;                          // original -> sorted (built on first use)
+    mutable int iperm_ok = 0;                           // iperm holds this build's inverse
     const void *pos_in = nullptr;                       // caller's positions (for fp64 resolution)
     // complete 8-ary tree in heap numbering: node x has children 8x+1..8x+8,
     // leaves are nodes first_leaf .. first_leaf + 8^depth - 1, leaf j holds
@@ -207,7 +216,7 @@ struct mhip_nlist {
     int k = 0;
     uint32_t *qsorted = nullptr;  // sorted-space particle index of each query slot
     uint32_t *qrow = nullptr;     // output row of each query slot
-    uint32_t *rinv = nullptr;     // query slot of each output row (masked lists)
+    mhip::RowRec *rowrec = nullptr;  // per output row: h, rho, vol, status, staged for the row outputs
     uint32_t *idx = nullptr;      // column-major [k][nq] sorted-space neighbour indices
     float *d2 = nullptr;          // column-major [k][nq] squared distances
     float *tau = nullptr;         // k-th squared distance per slot

@@ -311,8 +311,7 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const PT *__restrict__ po
                                                       float *__restrict__ sx, float *__restrict__ sy,
                                                       float *__restrict__ sz, float *__restrict__ lx,
                                                       float *__restrict__ ly, float *__restrict__ lz,
-                                                      uint32_t *__restrict__ perm,
-                                                      uint32_t *__restrict__ iperm) {
+                                                      uint32_t *__restrict__ perm) {
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= npad) return;
     const float nan = __builtin_nanf("");
@@ -320,7 +319,6 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const PT *__restrict__ po
     if (p < n) {
         uint32_t o = order[p];
         perm[p] = o;
-        iperm[o] = (uint32_t)p;
         for (int c = 0; c < 3; c++) {
             if (c < dim) {
                 PT v = pos[(int64_t)o * dim + c];
@@ -443,6 +441,7 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
         if ((rc = grow(ctx, (void **)&ix->nodes, tot * 8 * sizeof(float)))) return rc;
         ix->node_cap = tot;
     }
+    ix->iperm_ok = 0;
     ix->n = n;
     ix->npad = npad;
     ix->dim = dim;
@@ -493,12 +492,11 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     const dim3 gp((unsigned)((npad + 255) / 256));
     if (pos_f64)
         hipLaunchKernelGGL(k_gather_sorted<double>, gp, b256, 0, ctx->stream, (const double *)pos, n,
-                           npad, dim, vs, ix->sx, ix->sy, ix->sz, ix->lx, ix->ly, ix->lz, ix->perm,
-                           ix->iperm);
+                           npad, dim, vs, ix->sx, ix->sy, ix->sz, ix->lx, ix->ly, ix->lz, ix->perm);
     else
         hipLaunchKernelGGL(k_gather_sorted<float>, gp, b256, 0, ctx->stream, (const float *)pos, n,
                            npad, dim, vs, ix->sx, ix->sy, ix->sz, (float *)nullptr, (float *)nullptr,
-                           (float *)nullptr, ix->perm, ix->iperm);
+                           (float *)nullptr, ix->perm);
     hipLaunchKernelGGL(k_leaf_boxes, dim3((unsigned)((nslots * LEAF + 255) / 256)), b256, 0, ctx->stream,
                        ix->sx, ix->sy, ix->sz, n, nleaf, nslots, ix->nodes + first_leaf * 8);
     for (int d = depth - 1; d >= 0; d--) {

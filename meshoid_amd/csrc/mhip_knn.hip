@@ -1022,37 +1022,60 @@ __global__ void k_qsetup_mask(const int64_t *__restrict__ qidx, int64_t nq,
 }
 
 __global__ void k_qsetup_finish(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-                                int64_t nq, uint32_t *__restrict__ qsorted, uint32_t *__restrict__ qrow,
-                                uint32_t *__restrict__ rinv) {
+                                int64_t nq, uint32_t *__restrict__ qsorted, uint32_t *__restrict__ qrow) {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= nq) return;
     qsorted[s] = (uint32_t)keys[s];
     qrow[s] = vals[s];
-    rinv[vals[s]] = (uint32_t)s;
 }
 
-// Row-order outputs from the slot-ordered results, one thread per output row
-// r (slot rslot[r]): h, density = k m / (V_d h^d), volume = m / density
-// (Meshoid.py:165-170) and the status bits.  The writes are coalesced; the
-// slot reads are gathers, which the slots of nearby rows share.
-__global__ __launch_bounds__(256) void k_rows_out(const uint32_t *__restrict__ rslot, int64_t nq, int k, int dim,
+// inverse of the index's sorted -> original map (built on first use: only
+// queries on a subset of the particles need it)
+__global__ __launch_bounds__(256) void k_iperm(const uint32_t *__restrict__ perm, int64_t n,
+                                              uint32_t *__restrict__ iperm) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p < n) iperm[perm[p]] = (uint32_t)p;
+}
+
+static int ensure_iperm(mhip_ctx *ctx, const mhip_index *ix) {
+    if (ix->iperm_ok) return 0;
+    hipLaunchKernelGGL(k_iperm, dim3((unsigned)((ix->n + 255) / 256)), dim3(256), 0, ctx->stream, ix->perm, ix->n,
+                       ix->iperm);
+    ix->iperm_ok = 1;
+    return check_hip(ctx, hipGetLastError(), "inverse permutation");
+}
+
+// Row-order outputs from the slot-ordered results.  One thread per query
+// slot forms h, density = k m / (V_d h^d), volume = m / density
+// (Meshoid.py:165-170) and the status bits, and writes them as one 16-byte
+// record to its output row (row = srow[slot]: one scattered write per row);
+// a second pass copies the records, in row order, into the four output
+// arrays (coalesced).  The slot arrays are read in order, so no inverse of
+// the sort order is needed.
+__global__ __launch_bounds__(256) void k_rows_rec(const uint32_t *__restrict__ srow, int64_t nq, int k, int dim,
                                                  const double *__restrict__ h64, const int32_t *__restrict__ st,
-                                                 const double *__restrict__ m_rows, float *__restrict__ h_out,
-                                                 float *__restrict__ rho_out, float *__restrict__ vol_out,
-                                                 int32_t *__restrict__ status_out) {
-    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (r >= nq) return;
-    const uint32_t s = rslot[r];
-    if (status_out) status_out[r] = st[s];
-    if (!h_out && !rho_out && !vol_out) return;
+                                                 const double *__restrict__ m_rows, RowRec *__restrict__ rec) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nq) return;
+    const uint32_t r = srow[s];
     const double h = h64[s];
     const double vn = dim == 3 ? 4.18879020478639098 : (dim == 2 ? 3.14159265358979324 : 2.0);
     const double m = m_rows ? m_rows[r] : 1.0;
     const double hp = dim == 3 ? h * h * h : (dim == 2 ? h * h : h);
     const double rho = (double)k * m / (vn * hp);
-    if (h_out) h_out[r] = (float)h;
-    if (rho_out) rho_out[r] = (float)rho;
-    if (vol_out) vol_out[r] = (float)(m / rho);
+    rec[r] = RowRec{(float)h, (float)rho, (float)(m / rho), st[s]};
+}
+
+__global__ __launch_bounds__(256) void k_rows_split(const RowRec *__restrict__ rec, int64_t nq,
+                                                   float *__restrict__ h_out, float *__restrict__ rho_out,
+                                                   float *__restrict__ vol_out, int32_t *__restrict__ status_out) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= nq) return;
+    const RowRec v = rec[r];
+    if (h_out) h_out[r] = v.h;
+    if (rho_out) rho_out[r] = v.rho;
+    if (vol_out) vol_out[r] = v.vol;
+    if (status_out) status_out[r] = v.status;
 }
 
 static int nl_grow(mhip_ctx *ctx, void **p, size_t bytes) {
@@ -1080,10 +1103,11 @@ static void launch_main(const KnnArgs &a, bool per, bool p64, dim3 g, hipStream_
 static int nl_reserve(mhip_ctx *ctx, mhip_nlist *nl, int64_t nq, int k) {
     int rc = 0;
     if (nl->cap < nq || nl->k < k) {
-        void **arr[] = {(void **)&nl->qsorted, (void **)&nl->qrow, (void **)&nl->rinv, (void **)&nl->tau,
+        void **arr[] = {(void **)&nl->qsorted, (void **)&nl->qrow, (void **)&nl->tau,
                         (void **)&nl->nlt,     (void **)&nl->h,    (void **)&nl->status};
         for (auto p : arr)
             if ((rc = nl_grow(ctx, p, nq * 4))) return rc;
+        if ((rc = nl_grow(ctx, (void **)&nl->rowrec, nq * 16))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->amb, (nq + 1) * 4))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->h64, nq * 8))) return rc;
         if ((rc = nl_grow(ctx, (void **)&nl->idx, (size_t)nq * k * 4))) return rc;
@@ -1104,6 +1128,7 @@ static int query_setup(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx,
     if (!s) return rc;
     uint64_t *k0 = (uint64_t *)s, *k1 = k0 + nq;
     uint32_t *v0 = (uint32_t *)(k1 + nq), *v1 = v0 + nq, *counts = v1 + nq;
+    if ((rc = ensure_iperm(ctx, ix))) return rc;
     hipLaunchKernelGGL(k_qsetup_mask, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream, qidx, nq,
                        ix->iperm, k0, v0);
     int bits = 1;
@@ -1112,7 +1137,7 @@ static int query_setup(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx,
     uint32_t *vs;
     if ((rc = radix_sort_pairs(ctx, k0, v0, k1, v1, counts, nq, bits, &ks, &vs))) return rc;
     hipLaunchKernelGGL(k_qsetup_finish, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream, ks, vs,
-                       nq, nl->qsorted, nl->qrow, nl->rinv);
+                       nq, nl->qsorted, nl->qrow);
     return 0;
 }
 
@@ -1243,9 +1268,10 @@ int knn_run(mhip_ctx *ctx, const mhip_index *ix, const int64_t *qidx, int64_t nq
     pr3.end();
     if (status_out || h_out || rho_out || vol_out) {
         ProfRegion pr4(ctx, "knn_rows_out");
-        hipLaunchKernelGGL(k_rows_out, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s,
-                           masked ? nl->rinv : ix->iperm, nq, k, ix->dim, nl->h64, nl->status, m_rows, h_out,
-                           rho_out, vol_out, status_out);
+        const dim3 gq((unsigned)((nq + 255) / 256));
+        hipLaunchKernelGGL(k_rows_rec, gq, dim3(256), 0, s, masked ? nl->qrow : ix->perm, nq, k, ix->dim, nl->h64,
+                           nl->status, m_rows, nl->rowrec);
+        hipLaunchKernelGGL(k_rows_split, gq, dim3(256), 0, s, nl->rowrec, nq, h_out, rho_out, vol_out, status_out);
     }
     return check_hip(ctx, hipGetLastError(), "knn launch");
 }
