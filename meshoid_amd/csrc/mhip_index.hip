@@ -47,6 +47,67 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const PT *__restrict__ pos
     if (threadIdx.x < 7) part[blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// The same for 3D float32 positions read as 16-byte vectors of the flat
+// (n, 3) array: with a thread count divisible by 3, thread t always meets
+// components (t, t + 1, t + 2, t) mod 3 in its vectors, so it keeps three
+// rotated accumulators and maps them back to x, y, z at the end.  The at most
+// 3 floats after the last whole vector go to thread 0.  pos must be 16-byte
+// aligned.
+__global__ __launch_bounds__(256) void k_bbox_partial_v3(const float *__restrict__ pos, int64_t n,
+                                                        float *__restrict__ part) {
+    const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nt = (int64_t)gridDim.x * blockDim.x;  // a multiple of 3
+    const int64_t nv = 3 * n / 4;
+    const float4 *v4 = reinterpret_cast<const float4 *>(pos);
+    float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float nanflag = 0.f;
+    for (int64_t v = gt; v < nv; v += nt) {
+        const float4 x = v4[v];
+        if (x.x != x.x || x.y != x.y || x.z != x.z || x.w != x.w) nanflag = 1.f;
+        rl[0] = fminf(rl[0], fminf(x.x, x.w));
+        rh[0] = fmaxf(rh[0], fmaxf(x.x, x.w));
+        rl[1] = fminf(rl[1], x.y);
+        rh[1] = fmaxf(rh[1], x.y);
+        rl[2] = fminf(rl[2], x.z);
+        rh[2] = fmaxf(rh[2], x.z);
+    }
+    const int c0 = (int)(gt % 3);
+    float lo[3], hi[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int r = (c - c0 + 3) % 3;  // accumulator of component c
+        lo[c] = r == 0 ? rl[0] : (r == 1 ? rl[1] : rl[2]);
+        hi[c] = r == 0 ? rh[0] : (r == 1 ? rh[1] : rh[2]);
+    }
+    if (gt == 0) {
+        for (int64_t e = 4 * nv; e < 3 * n; e++) {
+            const float x = pos[e];
+            const int c = (int)(e % 3);
+            if (x != x) nanflag = 1.f;
+            lo[c] = fminf(lo[c], x);
+            hi[c] = fmaxf(hi[c], x);
+        }
+    }
+    __shared__ float s[7][256];
+    for (int c = 0; c < 3; c++) {
+        s[c][threadIdx.x] = lo[c];
+        s[3 + c][threadIdx.x] = hi[c];
+    }
+    s[6][threadIdx.x] = nanflag;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int c = 0; c < 3; c++) {
+                s[c][threadIdx.x] = fminf(s[c][threadIdx.x], s[c][threadIdx.x + w]);
+                s[3 + c][threadIdx.x] = fmaxf(s[3 + c][threadIdx.x], s[3 + c][threadIdx.x + w]);
+            }
+            s[6][threadIdx.x] = fmaxf(s[6][threadIdx.x], s[6][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) part[blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
+}
+
 // single block: reduce partials into out[0..6] (lo xyz, hi xyz, nan flag) and
 // the quantisation frame frame[0..3] (origin xyz, scale)
 __global__ __launch_bounds__(256) void k_bbox_final(const float *__restrict__ part, int nb, int dim,
@@ -463,12 +524,16 @@ int index_build(mhip_ctx *ctx, const void *pos, int pos_f64, int64_t n, int dim,
     uint32_t *v0 = (uint32_t *)(s + off_v0), *v1 = (uint32_t *)(s + off_v1);
     uint32_t *counts = (uint32_t *)(s + off_c);
     float *part = (float *)(s + off_bb), *bb = rec ? rec : part + nbb * 8;
-    const int nb = (int)std::min<int64_t>(nbb, (n + 255) / 256);
+    // 16-byte loads for aligned 3D float32 sets; 1020 blocks (a multiple of 3)
+    const bool v3 = !pos_f64 && dim == 3 && n >= ((int64_t)1 << 18) && ((uintptr_t)pos & 15) == 0;
+    const int nb = v3 ? 1020 : (int)std::min<int64_t>(nbb, (n + 255) / 256);
     const dim3 g1((unsigned)((n + 255) / 256)), b256(256);
     ProfRegion pr_bb(ctx, "index_bbox");
     if (pos_f64) {
         hipLaunchKernelGGL(k_bbox_partial<double>, dim3(nb), b256, 0, ctx->stream, (const double *)pos,
                            n, dim, part);
+    } else if (v3) {
+        hipLaunchKernelGGL(k_bbox_partial_v3, dim3(nb), b256, 0, ctx->stream, (const float *)pos, n, part);
     } else {
         hipLaunchKernelGGL(k_bbox_partial<float>, dim3(nb), b256, 0, ctx->stream, (const float *)pos,
                            n, dim, part);

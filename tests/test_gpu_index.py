@@ -39,6 +39,9 @@ def test_sort_is_stable_hilbert_order(n):
     ix = engine.SpatialIndex().build(torch.from_numpy(pos).cuda())
     perm = ix.order().cpu().numpy()
     assert np.array_equal(np.sort(perm), np.arange(n))
+    # the bounding box exactly (16-byte vector reads from 2^18 particles on)
+    lo, hi = np.asarray(ix.bbox()[:3], np.float32), np.asarray(ix.bbox()[3:6], np.float32)
+    assert np.array_equal(lo, pos.min(0)) and np.array_equal(hi, pos.max(0))
     keys = host_keys(pos, ix.bbox())[perm]
     assert (keys[1:] >= keys[:-1]).all()
     tie = keys[1:] == keys[:-1]
@@ -59,3 +62,22 @@ def test_sort_order_low_dimensions(dim):
     assert np.array_equal(np.sort(perm), np.arange(len(pos)))
     keys = host_keys(pos, ix.bbox(), dim)[perm]
     assert (keys[1:] >= keys[:-1]).all()
+
+
+@pytest.mark.parametrize("where", ["middle", "tail"])
+def test_nan_position_rejected_large(where):
+    """A NaN coordinate fails the build also where the box is read in 16-byte
+    vectors (n = 300001: the last particle's coordinates are the 3 floats after
+    the last whole vector)."""
+    torch = _torch()
+    from meshoid_amd import engine
+
+    print("nan", where, flush=True)
+    n = 300001
+    pos = np.random.default_rng(5).random((n, 3)).astype(np.float32)
+    if where == "middle":
+        pos[123457, 1] = np.nan
+    else:
+        pos[n - 1, 2] = np.nan
+    with pytest.raises(ValueError):
+        engine.SpatialIndex().build(torch.from_numpy(pos).cuda())
