@@ -244,8 +244,10 @@ MHIP_DEV void apply_row(const ApplyArgs &a, int64_t s, int fb, int fe, int gl = 
 
 // apply_row for the common case of one float32 field, 3D, first order,
 // float32 positions and the packed (x, y, z, f) copy (a.K == 1, a.xyzf set,
-// a.k >= 1): the same sums in the same order (results are bit-identical), with
-// the loads software-pipelined.  The generic loop waits on a neighbour's
+// a.k >= 1): the same float64 terms as the generic loop (at the same lane
+// count the results are bit-identical; with four lanes here against two there
+// the partial sums are added in another order), with the loads
+// software-pipelined.  The generic loop waits on a neighbour's
 // index and then on its position before it can sum it, so every neighbour
 // costs two memory round trips with nothing else in flight in the wavefront;
 // here the position of the next neighbour and the index of the one after it
